@@ -1,0 +1,224 @@
+"""Seeded SDP instance generators in HDSDP's conic input format.
+
+The solver takes one SDP block as a CSC matrix with ``m + 1`` columns over the
+packed-lower row index of the ``n x n`` symmetric coefficient matrices: column 0
+holds ``C`` and column ``i + 1`` holds ``A_i`` (``interface/def_hdsdp_user_data.h:11-21``,
+``interface/hdsdp_conic_sdp.c:1367-1378``).  Packed-lower, column-major indexing is
+``idx(i, j) = (2n - j - 1) j / 2 + i`` for ``i >= j`` (``interface/hdsdp_utils.h:48-51``).
+
+Every generator draws from ``numpy.random.default_rng(seed)`` (PCG64), so the same
+seed gives bit-identical instances on any host with the same numpy.
+"""
+from __future__ import annotations
+
+import dataclasses
+import struct
+
+import numpy as np
+
+__all__ = [
+    "SDPInstance",
+    "packed_index",
+    "maxcut",
+    "lovasz_theta",
+    "dense_random",
+    "rank_one",
+    "write_instance",
+    "read_instance",
+    "CONFIGS",
+]
+
+
+@dataclasses.dataclass
+class SDPInstance:
+    """One SDP block ``min <C, X> s.t. <A_i, X> = b_i, X psd``.
+
+    ``beg``/``idx``/``elem`` is the CSC form described in the module docstring.
+    ``factor`` is set instead for the factor-form rank-one input (``A_i = s_i a_i a_i^T``),
+    where ``factor`` is ``n x m`` column-major and ``sign`` holds ``s_i``; then
+    ``beg``/``idx``/``elem`` hold only column 0 (``C``).
+    """
+
+    name: str
+    n: int
+    m: int
+    beg: np.ndarray  # int64, m + 2 entries
+    idx: np.ndarray  # int32 packed-lower row indices
+    elem: np.ndarray  # float64
+    b: np.ndarray  # float64, m
+    factor: np.ndarray | None = None  # float64, (n, m) Fortran order
+    sign: np.ndarray | None = None  # float64, m
+
+    @property
+    def nnz(self) -> int:
+        return int(self.beg[-1])
+
+    def dense_coeff(self, col: int) -> np.ndarray:
+        """Full symmetric ``n x n`` matrix of column ``col`` (0 = C, i + 1 = A_i)."""
+        n = self.n
+        out = np.zeros((n, n))
+        if self.factor is not None and col > 0:
+            a = self.factor[:, col - 1]
+            return self.sign[col - 1] * np.outer(a, a)
+        lo, hi = int(self.beg[col]), int(self.beg[col + 1])
+        jj, ii = _unpack(n, self.idx[lo:hi].astype(np.int64))
+        out[ii, jj] = self.elem[lo:hi]
+        out[jj, ii] = self.elem[lo:hi]
+        return out
+
+
+def packed_index(n: int, i, j):
+    """Packed-lower column-major index of entry ``(i, j)`` with ``i >= j``."""
+    i = np.asarray(i, dtype=np.int64)
+    j = np.asarray(j, dtype=np.int64)
+    return (2 * n - j - 1) * j // 2 + i
+
+
+def _unpack(n: int, p: np.ndarray):
+    """Inverse of :func:`packed_index`: returns ``(j, i)`` (column, row)."""
+    p = np.asarray(p, dtype=np.int64)
+    starts = packed_index(n, np.arange(n), np.arange(n))  # diagonal positions
+    j = np.searchsorted(starts, p, side="right") - 1
+    i = p - starts[j] + j
+    return j, i
+
+
+def _assemble(name, n, cols, b):
+    """Build CSC arrays from a list of (packed_idx, values) per column."""
+    beg = np.zeros(len(cols) + 1, dtype=np.int64)
+    for k, (pi, _) in enumerate(cols):
+        beg[k + 1] = beg[k] + len(pi)
+    idx = np.empty(int(beg[-1]), dtype=np.int32)
+    elem = np.empty(int(beg[-1]), dtype=np.float64)
+    for k, (pi, pv) in enumerate(cols):
+        order = np.argsort(pi, kind="stable")
+        idx[beg[k]:beg[k + 1]] = np.asarray(pi, dtype=np.int64)[order]
+        elem[beg[k]:beg[k + 1]] = np.asarray(pv, dtype=np.float64)[order]
+    return SDPInstance(name, n, len(cols) - 1, beg, idx, elem, np.asarray(b, dtype=np.float64))
+
+
+def _random_graph(rng, n, p):
+    iu, ju = np.triu_indices(n, 1)  # i < j
+    keep = rng.random(iu.shape[0]) < p
+    return iu[keep], ju[keep]
+
+
+def maxcut(n: int, p: float, seed: int, weights: str = "pm1") -> SDPInstance:
+    """MaxCut relaxation on G(n, p): ``C = -L/4``, ``diag(X) = 1``.
+
+    ``weights`` is ``"pm1"`` (uniform on {-1, +1}) or ``"unif"`` (U(0, 1)).
+    """
+    rng = np.random.default_rng(seed)
+    i, j = _random_graph(rng, n, p)
+    if weights == "pm1":
+        w = rng.choice(np.array([-1.0, 1.0]), size=i.shape[0])
+    elif weights == "unif":
+        w = rng.random(i.shape[0])
+    else:
+        raise ValueError(weights)
+    deg = np.zeros(n)
+    np.add.at(deg, i, w)
+    np.add.at(deg, j, w)
+    # C = -(D - W)/4: diagonal -deg/4, off-diagonal +w/4 at (max, min)
+    rows = np.concatenate([np.arange(n), j])  # j > i so (j, i) is lower
+    colsj = np.concatenate([np.arange(n), i])
+    vals = np.concatenate([-deg / 4.0, w / 4.0])
+    nz = vals != 0.0
+    cpk = packed_index(n, rows[nz], colsj[nz])
+    cols = [(cpk, vals[nz])]
+    for k in range(n):
+        cols.append((np.array([packed_index(n, k, k)]), np.array([1.0])))
+    return _assemble(f"maxcut_n{n}_p{p}_s{seed}", n, cols, np.ones(n))
+
+
+def lovasz_theta(n: int, p: float, seed: int) -> SDPInstance:
+    """Lovasz theta SDP on G(n, p): ``C = -J``, ``tr X = 1``, ``X_ij = 0`` per edge."""
+    rng = np.random.default_rng(seed)
+    i, j = _random_graph(rng, n, p)
+    npk = n * (n + 1) // 2
+    cols = [(np.arange(npk), -np.ones(npk))]
+    cols.append((packed_index(n, np.arange(n), np.arange(n)), np.ones(n)))
+    for a, c in zip(i, j):
+        cols.append((np.array([packed_index(n, c, a)]), np.array([1.0])))
+    b = np.zeros(len(cols) - 1)
+    b[0] = 1.0
+    return _assemble(f"theta_n{n}_p{p}_s{seed}", n, cols, b)
+
+
+def dense_random(n: int, m: int, seed: int) -> SDPInstance:
+    """Strictly feasible dense SDP: ``A_i`` packed N(0,1)/sqrt(n), ``C = sum y0_i A_i + I``,
+    ``b_i = tr(A_i)`` (so ``X = I`` and ``y0`` are interior)."""
+    rng = np.random.default_rng(seed)
+    npk = n * (n + 1) // 2
+    diag = packed_index(n, np.arange(n), np.arange(n))
+    beg = np.arange(m + 2, dtype=np.int64) * npk
+    idx = np.tile(np.arange(npk, dtype=np.int32), m + 1)
+    elem = np.empty((m + 1) * npk)
+    y0 = rng.standard_normal(m)
+    cvec = np.zeros(npk)
+    b = np.empty(m)
+    scale = 1.0 / np.sqrt(n)
+    for k in range(m):
+        a = rng.standard_normal(npk) * scale
+        elem[(k + 1) * npk:(k + 2) * npk] = a
+        b[k] = a[diag].sum()
+        cvec += y0[k] * a
+    cvec[diag] += 1.0
+    elem[:npk] = cvec
+    return SDPInstance(f"dense_n{n}_m{m}_s{seed}", n, m, beg, idx, elem, b)
+
+
+def rank_one(n: int, m: int, r: int, seed: int) -> SDPInstance:
+    """Rank-one constraints ``A_k = a_k a_k^T``, ``a_k ~ N(0, I/n)``;
+    ``b_k = a_k^T X0 a_k`` with ``X0 = Z Z^T / r + 1e-3 I``; ``C = I``.
+
+    Returned in factor form (``factor``/``sign``); ``C`` is the only CSC column.
+    """
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal((n, m)) / np.sqrt(n)
+    z = rng.standard_normal((n, r))
+    za = z.T @ a  # r x m
+    b = (za * za).sum(axis=0) / r + 1e-3 * (a * a).sum(axis=0)
+    diag = packed_index(n, np.arange(n), np.arange(n))
+    beg = np.zeros(m + 2, dtype=np.int64)
+    beg[1:] = n
+    inst = SDPInstance(f"rankone_n{n}_m{m}_s{seed}", n, m, beg, diag.astype(np.int32),
+                       np.ones(n), b, factor=np.asfortranarray(a), sign=np.ones(m))
+    return inst
+
+
+# The five benchmark configurations (seed = configuration number).
+CONFIGS = {
+    "C1": lambda: maxcut(200, 0.05, seed=1, weights="pm1"),
+    "C2": lambda: lovasz_theta(400, 0.1, seed=2),
+    "C3": lambda: dense_random(1000, 2000, seed=3),
+    "C4": lambda: rank_one(2000, 20000, 20, seed=4),
+    "C5": lambda: maxcut(2000, 0.01, seed=5, weights="unif"),
+}
+
+_MAGIC = b"HDSDPIN1"
+
+
+def write_instance(inst: SDPInstance, path: str) -> None:
+    """Binary instance file: magic, n, m, nnz, beg[m+2] (i64), idx (i32), elem (f64), b (f64)."""
+    if inst.factor is not None:
+        raise ValueError("factor-form instances have no CSC file form")
+    with open(path, "wb") as f:
+        f.write(_MAGIC)
+        f.write(struct.pack("<iiq", inst.n, inst.m, inst.nnz))
+        f.write(np.ascontiguousarray(inst.beg, dtype="<i8").tobytes())
+        f.write(np.ascontiguousarray(inst.idx, dtype="<i4").tobytes())
+        f.write(np.ascontiguousarray(inst.elem, dtype="<f8").tobytes())
+        f.write(np.ascontiguousarray(inst.b, dtype="<f8").tobytes())
+
+
+def read_instance(path: str, name: str = "file") -> SDPInstance:
+    with open(path, "rb") as f:
+        if f.read(8) != _MAGIC:
+            raise ValueError("not an instance file")
+        n, m, nnz = struct.unpack("<iiq", f.read(16))
+        beg = np.frombuffer(f.read(8 * (m + 2)), dtype="<i8").copy()
+        idx = np.frombuffer(f.read(4 * nnz), dtype="<i4").copy()
+        elem = np.frombuffer(f.read(8 * nnz), dtype="<f8").copy()
+        b = np.frombuffer(f.read(8 * m), dtype="<f8").copy()
+    return SDPInstance(name, n, m, beg, idx, elem, b)

@@ -1,0 +1,161 @@
+/*
+ * hdsdp_gpu.h -- C ABI of the MI355X dual-scaling hot path.
+ *
+ * One `hdsdp_dev` holds one dense SDP block resident in HBM: its coefficient
+ * matrices, the three dual buffers S (DUALVAR), trial S (DUALCHECK) and dS
+ * (DUALSTEP), their Cholesky factors, S^-1, the Schur matrix M and its
+ * preconditioner.  Only m-vectors and scalars cross the host boundary.
+ *
+ * Each entry point replaces one slot of the reference's plugin tables; the
+ * citation names the slot and the implementation it stands in for.  Plain
+ * pointers and sizes only.  Buffer ids are the reference's
+ * (interface/hdsdp_conic.h:24-26): 0 = DUALVAR, 1 = DUALCHECK, 2 = DUALSTEP.
+ * Schur types (interface/hdsdp_conic.h:16-19): 0 = INFEASIBLE, 1 = CORRECTOR,
+ * 2 = HOMOGENEOUS, 3 = PRIMAL.  All host arrays are FP64.
+ *
+ * Errors: HDSDP_RETCODE_FAILED for any device error or unsupported request;
+ * a matrix that is not positive definite is not an error (isPsd = 0), as in
+ * linalg/hdsdp_linsolver.c:535-545.
+ */
+#ifndef HDSDP_AMD_HDSDP_GPU_H
+#define HDSDP_AMD_HDSDP_GPU_H
+
+#include "hdsdp.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Coefficient matrix kinds after presolve (linalg/def_hdsdp_sdpdata.h:25-33). */
+#define HDSDP_COEFF_ZERO   0
+#define HDSDP_COEFF_SPARSE 1
+#define HDSDP_COEFF_DENSE  2
+#define HDSDP_COEFF_SPR1   3
+#define HDSDP_COEFF_DSR1   4
+
+/* One presolved coefficient matrix (linalg/def_hdsdp_sdpdata.h:78-141).
+ *   SPARSE: nnz lower triplets (row >= col, 0-based) in row/col/val.
+ *   DENSE : val = packed-lower column-major n(n+1)/2 entries.
+ *   SPR1  : A = sign a a^T, a has nnz entries at rows `row`, values `val`.
+ *   DSR1  : A = sign a a^T, val = dense a (n entries).
+ *   ZERO  : nothing. */
+typedef struct {
+    int type;
+    int nnz;
+    const int *row;
+    const int *col;
+    const double *val;
+    double sign;
+} hdsdp_coeff_view;
+
+typedef struct hdsdp_dev hdsdp_dev;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+
+/* Allocate device state for an n x n block with m constraints.  precision:
+ * HDSDP_PRECISION_FP64 or HDSDP_PRECISION_MIXED (include/hdsdp.h).
+ * Replaces sdpDenseConeCreateImpl + the dense-direct HFpLinsysCreate calls
+ * (interface/hdsdp_conic_sdp.c:1319-1333, 100-111) and HKKTIAllocDenseKKT
+ * (interface/hdsdp_schur.c:11-44). */
+hdsdp_retcode hdsdp_dev_create( hdsdp_dev **pdev, int nRow, int nCol, int precision );
+
+/* Upload m + 1 presolved coefficients (coeffs[0] = C, coeffs[i + 1] = A_i).
+ * Replaces the data half of sdpDenseConeProcDataImpl/PresolveImpl
+ * (interface/hdsdp_conic_sdp.c:1352-1514). */
+hdsdp_retcode hdsdp_dev_load( hdsdp_dev *dev, const hdsdp_coeff_view *coeffs );
+
+/* Rescale C in place (coneScal slot: interface/hdsdp_conic_sdp.c:1600-1604). */
+hdsdp_retcode hdsdp_dev_scale_obj( hdsdp_dev *dev, double dScal );
+
+void hdsdp_dev_destroy( hdsdp_dev **pdev );
+
+/* ---- dual buffers (hdsdp_cone slots for DENSE_SDP) ---------------------- */
+
+/* target = dACoefScal * sum_i dACoef[i] A_i + dCCoef * C + dEyeCoef * I.
+ * The caller folds the cone perturbation into dEyeCoef.
+ * Replaces sdpDenseConeIUpdateBuffer (interface/hdsdp_conic_sdp.c:343-402). */
+hdsdp_retcode hdsdp_dev_update_buffer( hdsdp_dev *dev, double dCCoef, double dACoefScal,
+                                       const double *dACoef, double dEyeCoef, int whichBuffer );
+
+/* Cholesky of buffer DUALVAR (into the dual factor) or DUALCHECK (into the
+ * checker factor); *isPsd = 1 iff positive definite.  Replaces
+ * HFpLinsysPsdCheck on lapack_flinsys (linalg/hdsdp_linsolver.c:525-560). */
+hdsdp_retcode hdsdp_dev_factorize( hdsdp_dev *dev, int whichBuffer, int *isPsd );
+
+/* DUALCHECK: checker = S + dStep * dS;  DUALVAR: S += dStep * dS; then factorize.
+ * Replaces sdpDenseConeAddStepToBufferAndCheck (interface/hdsdp_conic_sdp.c:2266-2295). */
+hdsdp_retcode hdsdp_dev_axpy_check( hdsdp_dev *dev, double dStep, int whichBuffer, int *isPsd );
+
+/* 2 * sum log L_ii of the factor of `whichBuffer`.  Replaces the factor half of
+ * sdpDenseConeGetBarrier (interface/hdsdp_conic_sdp.c:2185-2224). */
+hdsdp_retcode hdsdp_dev_logdet( hdsdp_dev *dev, int whichBuffer, double *logdet );
+
+/* Largest step a with S_w + a dS psd from the current DUALSTEP buffer and the
+ * factor of `whichBuffer`: Lanczos on L^-1 (-dS) L^-T (coneRatioTest slot;
+ * interface/hdsdp_conic_sdp.c:1638-1682, linalg/hdsdp_lanczos.c:161-292). */
+hdsdp_retcode hdsdp_dev_ratio_test( hdsdp_dev *dev, int whichBuffer, double *maxStep );
+
+/* ---- Schur complement (coneBuildSchur slot) ----------------------------- */
+
+/* Build with W = S^-1 from the dual factor (or W = X for PRIMAL):
+ *   M_ij = <A_i, W A_j W> (lower triangle, not for CORRECTOR),
+ *   dASinv[i] = <A_i, W>, dASinvRdSinv[i] = Rd <A_i, W W>,
+ *   *dTraceSinv = tr(W) when Rd != 0 (not for CORRECTOR), else 0.
+ * Replaces sdpDenseConeGetKKT and its strategies M2-M5
+ * (interface/hdsdp_conic_sdp.c:683-1052, 1722-1804). */
+hdsdp_retcode hdsdp_dev_schur( hdsdp_dev *dev, int typeKKT, double dualResidual,
+                               double *dASinv, double *dASinvRdSinv, double *dTraceSinv );
+
+/* ---- Schur solve (hdsdp_linsys_fp slots of the KKT solver) -------------- */
+
+/* CG parameters (conjGradLinSolverSetParam, linalg/hdsdp_linsolver.c:735-760). */
+hdsdp_retcode hdsdp_dev_kkt_params( hdsdp_dev *dev, double relTol, double absTol,
+                                    int maxIter, int nRestartFreq );
+/* M_ii += diag[i] (bound cone: interface/hdsdp_conic_bound.c:224-229). */
+hdsdp_retcode hdsdp_dev_kkt_add_diag( hdsdp_dev *dev, const double *diag );
+/* HKKTRegularize (interface/hdsdp_schur.c:348-373). */
+hdsdp_retcode hdsdp_dev_kkt_regularize( hdsdp_dev *dev, double dKKTReg );
+/* Preconditioner build: Jacobi diagonal or Cholesky of M once switched
+ * (conjGradLinSolverNumeric, linalg/hdsdp_linsolver.c:768-833). */
+hdsdp_retcode hdsdp_dev_kkt_factorize( hdsdp_dev *dev );
+/* PCG on M with the reference's restart and Jacobi-to-Cholesky switch
+ * (conjGradSolve, linalg/hdsdp_linsolver.c:855-997).  rhs and sol may alias. */
+hdsdp_retcode hdsdp_dev_kkt_solve( hdsdp_dev *dev, const double *rhs, double *sol );
+
+/* ---- primal recovery and solution checks -------------------------------- */
+
+/* X = mu (S^-1 + S^-1 (sum dy_i A_i) S^-1) with S = C - A^T y (+ perturbation)
+ * factored into the checker; *ok = 0 if S is not psd.  Replaces
+ * sdpDenseConeGetPrimal (interface/hdsdp_conic_sdp.c:2328-2379). */
+hdsdp_retcode hdsdp_dev_primal( hdsdp_dev *dev, double mu, const double *y, const double *dy, int *ok );
+/* From the last recovered X: AX[i] = <A_i, X>, <C, X>, <X, S>, tr(X)
+ * (coneATimesX / coneTraceCX, interface/hdsdp_conic_sdp.c:2453-2471). */
+hdsdp_retcode hdsdp_dev_primal_stats( hdsdp_dev *dev, double *AX, double *traceCX,
+                                      double *XdotS, double *traceX );
+/* Copy the last X (n x n, full) or S (n x n, symmetrized) to the host. */
+hdsdp_retcode hdsdp_dev_get_primal( hdsdp_dev *dev, double *X );
+hdsdp_retcode hdsdp_dev_get_dual( hdsdp_dev *dev, double *S );
+
+/* ---- adjoint (no counterpart in the reference) -------------------------- */
+
+/* Differentiates the barrier optimality conditions M(y) dy = ... at the final
+ * mu with the last Schur matrix: solves M lambda = dLdy, then
+ * dLdb = lambda / mu and dLdC = -S^-1 (sum_i lambda_i A_i) S^-1 (n x n full).
+ * Callers: autograd through the solver (hdsdp_amd.adjoint). */
+hdsdp_retcode hdsdp_dev_adjoint( hdsdp_dev *dev, double mu, const double *dLdy,
+                                 double *dLdb, double *dLdC );
+
+/* ---- statistics ---------------------------------------------------------- */
+
+/* Counters for measurement: [0] Schur builds, [1] Cholesky of S, [2] CG
+ * iterations, [3] Lanczos matvecs, [4] Cholesky of M, [5] last CG iterations. */
+hdsdp_retcode hdsdp_dev_stats( hdsdp_dev *dev, double *stats, int nStats );
+
+/* Name of the implementation ("hip-gfx950" for the product library). */
+const char *hdsdp_dev_backend_name( void );
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HDSDP_AMD_HDSDP_GPU_H */
