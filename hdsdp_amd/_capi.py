@@ -47,6 +47,7 @@ _API = {
     "hdsdp_dev_kkt_regularize": (C.c_int, [P_VOID, C.c_double]),
     "hdsdp_dev_kkt_factorize": (C.c_int, [P_VOID]),
     "hdsdp_dev_kkt_solve": (C.c_int, [P_VOID, P_DBL, P_DBL]),
+    "hdsdp_dev_get_schur": (C.c_int, [P_VOID, P_DBL]),
     "hdsdp_dev_primal": (C.c_int, [P_VOID, C.c_double, P_DBL, P_DBL, P_INT]),
     "hdsdp_dev_primal_stats": (C.c_int, [P_VOID, P_DBL, P_DBL, P_DBL, P_DBL]),
     "hdsdp_dev_get_primal": (C.c_int, [P_VOID, P_DBL]),

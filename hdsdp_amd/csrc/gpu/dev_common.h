@@ -1,0 +1,67 @@
+// dev_common.h -- shared declarations of the gfx950 kernels and launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+
+#define HIP_OK(expr)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            fprintf(stderr, "[hdsdp_amd] HIP error %s at %s:%d\n", hipGetErrorString(e_), \
+                    __FILE__, __LINE__);                                                \
+            return HDSDP_RETCODE_FAILED;                                                \
+        }                                                                               \
+    } while (0)
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+namespace hdk {
+
+constexpr int TILE = 64;   // GEMM output tile (rows and columns)
+constexpr int KSTEP = 16;  // GEMM k-slab staged through LDS
+constexpr int NB = 64;     // Cholesky / triangular-inverse block
+
+// C = alpha * op(A) * op(B) + beta * C over `batch` problems (column-major).
+// lower != 0 restricts the output to i >= j (tiles above the diagonal skipped).
+template <typename T>
+void gemm(hipStream_t s, bool transA, bool transB, bool lower, int M, int N, int K, T alpha,
+          const T *A, int lda, long strideA, const T *B, int ldb, long strideB, T beta, T *C,
+          int ldc, long strideC, int batch = 1);
+
+// In-place lower Cholesky of the n x n matrix A (lda), blocked by NB.
+// work: NB*NB scratch; info: device int, 0 on success else failing column + 1.
+template <typename T>
+void potrf(hipStream_t s, int n, T *A, int lda, T *work, int *info);
+
+// Linv = L^-1 for the lower-triangular n x n factor L (upper of Linv zeroed).
+// work: n * n scratch.
+template <typename T>
+void trtri(hipStream_t s, int n, const T *L, int ldl, T *Linv, int ldi, T *work);
+
+// y[j] = alpha * sum_{i in rows(j)} A[i + j*lda] x[i] + beta * y[j] for j < ncols,
+// rows(j) = [0, m) (tri 0), [j, m) (tri 1, lower), [0, j] (tri 2, upper).
+template <typename T>
+void coldot(hipStream_t s, int m, int ncols, int tri, T alpha, const T *A, int lda, const T *x, T beta, T *y);
+
+// A[upper] = A[lower]^T for the n x n matrix.
+template <typename T>
+void mirror_lower(hipStream_t s, int n, T *A, int lda);
+// B = A^T (n x n).
+template <typename T>
+void transpose(hipStream_t s, int n, const T *A, int lda, T *B, int ldb);
+// Zero the strictly upper triangle.
+template <typename T>
+void zero_upper(hipStream_t s, int n, T *A, int lda);
+
+void fill(hipStream_t s, double *x, long n, double v);
+void fill_f(hipStream_t s, float *x, long n, float v);
+void copy_lower(hipStream_t s, int n, const double *A, int lda, double *B, int ldb);
+void convert_lower_to_float(hipStream_t s, int n, const double *A, int lda, float *B, int ldb);
+void add_diag(hipStream_t s, int n, double *A, int lda, const double *d, double scalar);
+// out[0] = 2 * sum log A_jj
+void logdet_diag(hipStream_t s, int n, const double *A, int lda, double *out);
+
+}  // namespace hdk

@@ -1,0 +1,830 @@
+// hdsdp_dev.hip -- the hdsdp_dev_* C ABI (include/hdsdp_gpu.h) on one MI355X.
+//
+// All n x n and m x m state of the SDP block lives in HBM: the three dual
+// buffers, two Cholesky factors with their inverses (used as triangular
+// solves by GEMV), S^-1, the Schur matrix M and its preconditioner.  Host
+// control follows the reference routines cited per function; per call only
+// m-vectors and scalars cross the host boundary.
+#include "hdsdp_gpu.h"
+#include "dev_common.h"
+#include "schur.h"
+#include "vec.h"
+#include "eig_small.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+using hdk::DevCoeffs;
+
+struct hdsdp_dev {
+    int m, n, precision;
+    hipStream_t st;
+    std::vector<void *> allocs;
+    DevCoeffs c;
+    int cType;
+    std::vector<int> heavy;  // coefficient indices handled by the dense sandwich path
+    int hasZero;
+
+    double *S[3], *L[2], *Li[2], *LiT[2];
+    long facVer[2], liVer[2];
+    double *W, *W2, *X, *T1, *T2, *work;
+    double *coef_d, *vecA_d, *vecB_d, *sc_d;
+    int *info_d;
+    // pinned staging
+    double *hcoef, *hvecA, *hvecB, *hsc, *hvec_n, *hY;
+    int *hinfo;
+    bool stagingBusy;
+
+    // Lanczos (linalg/hdsdp_lanczos.c)
+    int lzMax, lzComputed;
+    double *V, *Hd, *lv, *lw, *lz1, *lz2, *lwarm, *ly, *mv1, *mv2;
+    std::vector<double> Hh;
+
+    // Schur system
+    double *M, *Mchol, *MLi, *MLiT;
+    float *Mf, *MLif, *MLiTf, *cf1, *cf2, *workf;
+    double *jac, *cr, *crn, *cp, *cz, *cMp, *cx, *crhs, *ctmp;
+    int useJacobi, paramUseJacobi, maxIter, nRestart;
+    double absTol, relTol;
+    double stats[6];
+};
+
+namespace {
+
+template <typename T>
+hdsdp_retcode dalloc(hdsdp_dev *d, T **p, size_t count) {
+    void *q = nullptr;
+    size_t bytes = count * sizeof(T);
+    if (bytes == 0) bytes = 8;
+    if (hipMalloc(&q, bytes) != hipSuccess) {
+        fprintf(stderr, "[hdsdp_amd] hipMalloc of %zu bytes failed\n", bytes);
+        return HDSDP_RETCODE_MEMORY;
+    }
+    d->allocs.push_back(q);
+    *p = (T *) q;
+    return HDSDP_RETCODE_OK;
+}
+
+template <typename T>
+hdsdp_retcode halloc(T **p, size_t count) {
+    void *q = nullptr;
+    if (hipHostMalloc(&q, (count ? count : 1) * sizeof(T), hipHostMallocDefault) != hipSuccess)
+        return HDSDP_RETCODE_MEMORY;
+    *p = (T *) q;
+    return HDSDP_RETCODE_OK;
+}
+
+#define DCALL(expr)                                   \
+    do {                                              \
+        hdsdp_retcode r_ = (expr);                    \
+        if (r_ != HDSDP_RETCODE_OK) return r_;        \
+    } while (0)
+
+hdsdp_retcode sync(hdsdp_dev *d) {
+    HIP_OK(hipStreamSynchronize(d->st));
+    d->stagingBusy = false;
+    return HDSDP_RETCODE_OK;
+}
+
+hdsdp_retcode check_launch(void) {
+    HIP_OK(hipGetLastError());
+    return HDSDP_RETCODE_OK;
+}
+
+// upload `count` doubles from host memory through the pinned staging `stage`
+hdsdp_retcode upload(hdsdp_dev *d, double *dst, const double *src, size_t count, double *stage) {
+    if (d->stagingBusy) DCALL(sync(d));
+    memcpy(stage, src, count * sizeof(double));
+    HIP_OK(hipMemcpyAsync(dst, stage, count * sizeof(double), hipMemcpyHostToDevice, d->st));
+    d->stagingBusy = true;
+    return HDSDP_RETCODE_OK;
+}
+
+hdsdp_retcode download(hdsdp_dev *d, double *dst, const double *src, size_t count, double *stage) {
+    HIP_OK(hipMemcpyAsync(stage, src, count * sizeof(double), hipMemcpyDeviceToHost, d->st));
+    DCALL(sync(d));
+    memcpy(dst, stage, count * sizeof(double));
+    return HDSDP_RETCODE_OK;
+}
+
+hdsdp_retcode read_scalars(hdsdp_dev *d) {
+    HIP_OK(hipMemcpyAsync(d->hsc, d->sc_d, hdk::SC_SLOTS * sizeof(double), hipMemcpyDeviceToHost, d->st));
+    return sync(d);
+}
+
+int fidx(int whichBuffer) { return whichBuffer == 0 ? 0 : 1; }
+
+hdsdp_retcode ensure_linv(hdsdp_dev *d, int f) {
+    if (d->liVer[f] == d->facVer[f]) return HDSDP_RETCODE_OK;
+    hdk::trtri<double>(d->st, d->n, d->L[f], d->n, d->Li[f], d->n, d->work);
+    hdk::transpose<double>(d->st, d->n, d->Li[f], d->n, d->LiT[f], d->n);
+    d->liVer[f] = d->facVer[f];
+    return check_launch();
+}
+
+// out = L^-1 (-dS) L^-T in for the factor f (interface/hdsdp_conic_sdp.c:462-505)
+void lz_mvec(hdsdp_dev *d, int f, const double *in, double *out) {
+    const int n = d->n;
+    hdk::coldot<double>(d->st, n, n, 1, 1.0, d->Li[f], n, in, 0.0, d->mv1);
+    hdk::coldot<double>(d->st, n, n, 0, -1.0, d->S[2], n, d->mv1, 0.0, d->mv2);
+    hdk::coldot<double>(d->st, n, n, 2, 1.0, d->LiT[f], n, d->mv2, 0.0, out);
+    d->stats[3] += 1;
+}
+
+// glibc rand start vector and warm-start perturbation (linalg/hdsdp_lanczos.c:33-53)
+void lz_prepare(int n, double *vVec) {
+    srand(n);
+    for (int i = 0; i < n; ++i) {
+        srand(rand());
+        vVec[i] = sqrt(sqrt((rand() % 1627))) * (rand() % 2 - 0.5);
+    }
+}
+
+void lz_perturb(int n, double *vVec) {
+    srand(n);
+    for (int i = 0; i < n; ++i) {
+        srand(rand());
+        vVec[i] += 1e-03 * sqrt(sqrt((rand() % 1627))) * (rand() % 2 - 0.5);
+    }
+}
+
+// HLanczosSolve (linalg/hdsdp_lanczos.c:161-292) with device mat-vecs
+hdsdp_retcode lanczos(hdsdp_dev *d, int f, double *maxStep) {
+    const int n = d->n, maxDim = d->lzMax, nH = maxDim + 1;
+    std::vector<double> &H = d->Hh;
+    std::fill(H.begin(), H.end(), 0.0);
+    if (d->lzComputed == 0) {
+        lz_prepare(n, d->hvec_n);
+    } else {
+        HIP_OK(hipMemcpyAsync(d->hvec_n, d->lwarm, n * sizeof(double), hipMemcpyDeviceToHost, d->st));
+        DCALL(sync(d));
+        lz_perturb(n, d->hvec_n);
+    }
+    hdk::fill(d->st, d->V, (long) n * nH, 0.0);
+    hdk::fill(d->st, d->Hd, (long) nH * nH, 0.0);
+    HIP_OK(hipMemcpyAsync(d->lw, d->hvec_n, n * sizeof(double), hipMemcpyHostToDevice, d->st));
+    hdk::lz_start(d->st, n, d->lw, d->lv, d->V);
+    const int check = std::min(maxDim / 5, 3);
+    double U[31 * 31], evals[2], Y[2 * 31];
+    for (int k = 0; k < maxDim; ++k) {
+        lz_mvec(d, f, d->lv, d->lw);
+        hdk::lz_step(d->st, n, k, d->lw, d->V, d->Hd, nH, d->lv, d->sc_d);
+        DCALL(read_scalars(d));
+        double alpha = d->hsc[hdk::LZ_ALPHA], normPres = d->hsc[hdk::LZ_NORM];
+        H[k + k * nH] = alpha;
+        if (normPres > 0.0) H[(k + 1) + k * nH] = H[k + (k + 1) * nH] = normPres;
+        if ((k + 1) % check == 0 || k > maxDim - 1 || normPres == 0.0) {
+            const int kp1 = k + 1;
+            for (int i = 0; i < kp1; ++i) memcpy(U + kp1 * i, &H[nH * i], sizeof(double) * kp1);
+            if (kp1 < 2 || hdsdp_eig_small_top(kp1, U, 2, evals, Y) != 0) return HDSDP_RETCODE_FAILED;
+            double resi = fabs(H[kp1 + k * nH] * Y[kp1 + k]);
+            if (resi < 1e-04 || k >= maxDim - 1) {
+                double e1 = evals[1], e2 = evals[0];
+                memcpy(d->hY, Y, sizeof(double) * 2 * kp1);
+                HIP_OK(hipMemcpyAsync(d->ly, d->hY, sizeof(double) * 2 * kp1, hipMemcpyHostToDevice, d->st));
+                hdk::lz_comb(d->st, n, kp1, d->V, d->ly + kp1, d->lz1);
+                lz_mvec(d, f, d->lz1, d->lz2);
+                HIP_OK(hipMemcpyAsync(d->lwarm, d->lz2, n * sizeof(double), hipMemcpyDeviceToDevice, d->st));
+                hdk::axpy_norm(d->st, n, -e1, d->lz1, d->lz2, d->sc_d, hdk::LZ_R1);
+                hdk::lz_comb(d->st, n, kp1, d->V, d->ly, d->lz2);
+                lz_mvec(d, f, d->lz2, d->lz1);
+                hdk::axpy_norm(d->st, n, -e1, d->lz2, d->lz1, d->sc_d, hdk::LZ_R2);
+                DCALL(read_scalars(d));
+                double r1 = d->hsc[hdk::LZ_R1], r2 = d->hsc[hdk::LZ_R2];
+                double diff = e1 - e2 - r2;
+                double gamma = (diff > 0) ? diff : 1e-16;
+                double r1sq = r1 * r1 / gamma;
+                gamma = std::min(r1, r1sq);
+                if (gamma < 1e-03 || gamma + e1 <= 0.5) {
+                    *maxStep = (gamma + e1 <= 0.0) ? HDSDP_INFINITY : 1.0 / (gamma + e1);
+                    break;
+                } else {
+                    if (normPres == 0.0) return HDSDP_RETCODE_FAILED;
+                    *maxStep = 1.0 / (gamma + e1);
+                }
+            }
+        }
+    }
+    d->lzComputed += 1;
+    return check_launch();
+}
+
+hdsdp_retcode build_precond(hdsdp_dev *d) {
+    const int m = d->m;
+    if (d->useJacobi) {
+        hdk::get_diag(d->st, m, d->M, m, d->jac);
+        return check_launch();
+    }
+    d->stats[4] += 1;
+    if (d->precision == HDSDP_PRECISION_MIXED) {
+        if (!d->Mf) {
+            DCALL(dalloc(d, &d->Mf, (size_t) m * m));
+            DCALL(dalloc(d, &d->MLif, (size_t) m * m));
+            DCALL(dalloc(d, &d->MLiTf, (size_t) m * m));
+            DCALL(dalloc(d, &d->cf1, (size_t) m));
+            DCALL(dalloc(d, &d->cf2, (size_t) m));
+            DCALL(dalloc(d, &d->workf, (size_t) m * m));
+        }
+        hdk::convert_lower_to_float(d->st, m, d->M, m, d->Mf, m);
+        hdk::potrf<float>(d->st, m, d->Mf, m, d->workf, d->info_d);
+        HIP_OK(hipMemcpyAsync(d->hinfo, d->info_d, sizeof(int), hipMemcpyDeviceToHost, d->st));
+        DCALL(sync(d));
+        if (*d->hinfo != 0) return HDSDP_RETCODE_FAILED;
+        hdk::trtri<float>(d->st, m, d->Mf, m, d->MLif, m, d->workf);
+        hdk::transpose<float>(d->st, m, d->MLif, m, d->MLiTf, m);
+        return check_launch();
+    }
+    if (!d->Mchol) {
+        DCALL(dalloc(d, &d->Mchol, (size_t) m * m));
+        DCALL(dalloc(d, &d->MLi, (size_t) m * m));
+        DCALL(dalloc(d, &d->MLiT, (size_t) m * m));
+    }
+    HIP_OK(hipMemcpyAsync(d->Mchol, d->M, sizeof(double) * (size_t) m * m, hipMemcpyDeviceToDevice, d->st));
+    hdk::potrf<double>(d->st, m, d->Mchol, m, d->MLiT, d->info_d);
+    HIP_OK(hipMemcpyAsync(d->hinfo, d->info_d, sizeof(int), hipMemcpyDeviceToHost, d->st));
+    DCALL(sync(d));
+    if (*d->hinfo != 0) return HDSDP_RETCODE_FAILED;
+    hdk::trtri<double>(d->st, m, d->Mchol, m, d->MLi, m, d->MLiT);
+    hdk::transpose<double>(d->st, m, d->MLi, m, d->MLiT, m);
+    return check_launch();
+}
+
+// z = P^-1 r (and optionally p = z)
+void apply_precond(hdsdp_dev *d, const double *r, double *z, double *p) {
+    const int m = d->m;
+    if (d->useJacobi) {
+        hdk::cg_jacobi(d->st, m, r, d->jac, z, p);
+        return;
+    }
+    if (d->precision == HDSDP_PRECISION_MIXED) {
+        hdk::d2f(d->st, m, r, d->cf1);
+        hdk::coldot<float>(d->st, m, m, 2, 1.0f, d->MLiTf, m, d->cf1, 0.0f, d->cf2);
+        hdk::coldot<float>(d->st, m, m, 1, 1.0f, d->MLif, m, d->cf2, 0.0f, d->cf1);
+        hdk::f2d(d->st, m, d->cf1, z);
+    } else {
+        hdk::coldot<double>(d->st, m, m, 2, 1.0, d->MLiT, m, r, 0.0, d->ctmp);
+        hdk::coldot<double>(d->st, m, m, 1, 1.0, d->MLi, m, d->ctmp, 0.0, z);
+    }
+    if (p) hdk::cg_jacobi(d->st, m, z, nullptr, p, nullptr);
+}
+
+void symv_M(hdsdp_dev *d, const double *x, double *y) {
+    hdk::coldot<double>(d->st, d->m, d->m, 0, 1.0, d->M, d->m, x, 0.0, y);
+}
+
+// conjGradSolve (linalg/hdsdp_linsolver.c:855-997) on crhs; result in cx.
+// Returns 0 on success, 1 if the solution is left unwritten (tiny rhs), -1 on failure.
+int cg_solve(hdsdp_dev *d) {
+    const int m = d->m;
+    d->useJacobi = d->paramUseJacobi;
+    const int restart = std::max(d->nRestart, 20);
+    hdk::cg_start(d->st, m, d->crhs, d->cr, d->cx, d->sc_d);
+    if (read_scalars(d) != HDSDP_RETCODE_OK) return -1;
+    double rnorm = d->hsc[hdk::CG_RNORM], bnorm = d->hsc[hdk::CG_BNORM];
+    double tol = std::min(d->absTol, bnorm * d->relTol);
+    tol = std::max(tol, 0.1 * d->absTol);
+    if (rnorm < tol) return 1;
+    apply_precond(d, d->cr, d->cz, d->cp);
+    symv_M(d, d->cp, d->cMp);
+    int iter = 0, status = 0;
+    for (iter = 0; iter < d->maxIter; ++iter) {
+        hdk::cg_step1(d->st, m, d->cz, d->cr, d->cp, d->cMp, d->cx, d->crn, d->sc_d);
+        d->stats[2] += 1;
+        if (iter % restart == 5 && d->useJacobi) {
+            symv_M(d, d->cx, d->ctmp);
+            hdk::cg_residual(d->st, m, d->crhs, d->ctmp, d->cr);
+            apply_precond(d, d->cr, d->cp, nullptr);
+            symv_M(d, d->cp, d->cMp);
+            apply_precond(d, d->cr, d->cz, nullptr);
+            continue;
+        }
+        apply_precond(d, d->crn, d->cz, nullptr);
+        hdk::cg_step2(d->st, m, d->crn, d->cz, d->cp, d->cr, d->sc_d);
+        symv_M(d, d->cp, d->cMp);
+        if (read_scalars(d) != HDSDP_RETCODE_OK) return -1;
+        rnorm = d->hsc[hdk::CG_RNORM];
+        if (rnorm != rnorm) return -1;
+        if (iter > 20 && rnorm > 0.01 * bnorm) {
+            status = 2;
+            break;
+        }
+        if (rnorm < tol) break;
+    }
+    d->stats[5] = iter;
+    if (iter >= d->maxIter || status == 2) {
+        if (d->useJacobi) {
+            d->useJacobi = 0;
+            d->paramUseJacobi = 0;
+            if (build_precond(d) != HDSDP_RETCODE_OK) return -1;
+            return cg_solve(d);
+        }
+        return -1;
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *hdsdp_dev_backend_name(void) { return "hip-gfx950"; }
+
+hdsdp_retcode hdsdp_dev_create(hdsdp_dev **pdev, int nRow, int nCol, int precision) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        fprintf(stderr, "[hdsdp_amd] no HIP device: the MI355X library cannot run here\n");
+        return HDSDP_RETCODE_FAILED;
+    }
+    hdsdp_dev *d = new hdsdp_dev();
+    *pdev = d;
+    d->m = nRow;
+    d->n = nCol;
+    d->precision = precision;
+    HIP_OK(hipStreamCreateWithFlags(&d->st, hipStreamNonBlocking));
+    const size_t n = nCol, m = nRow, nn = n * n;
+    for (int k = 0; k < 3; ++k) DCALL(dalloc(d, &d->S[k], nn));
+    for (int k = 0; k < 2; ++k) {
+        DCALL(dalloc(d, &d->L[k], nn));
+        DCALL(dalloc(d, &d->Li[k], nn));
+        DCALL(dalloc(d, &d->LiT[k], nn));
+        d->facVer[k] = 0;
+        d->liVer[k] = -1;
+    }
+    DCALL(dalloc(d, &d->W, nn));
+    DCALL(dalloc(d, &d->W2, nn));
+    DCALL(dalloc(d, &d->X, nn));
+    DCALL(dalloc(d, &d->T1, nn));
+    DCALL(dalloc(d, &d->T2, nn));
+    DCALL(dalloc(d, &d->coef_d, m + 1));
+    DCALL(dalloc(d, &d->vecA_d, m + 1));
+    DCALL(dalloc(d, &d->vecB_d, m + 1));
+    DCALL(dalloc(d, &d->sc_d, hdk::SC_SLOTS));
+    DCALL(dalloc(d, &d->info_d, 1));
+    DCALL(halloc(&d->hcoef, m + 1));
+    DCALL(halloc(&d->hvecA, m + 1));
+    DCALL(halloc(&d->hvecB, m + 1));
+    DCALL(halloc(&d->hsc, hdk::SC_SLOTS));
+    DCALL(halloc(&d->hvec_n, n));
+    DCALL(halloc(&d->hY, 64));
+    DCALL(halloc(&d->hinfo, 1));
+    d->lzMax = 30;
+    d->lzComputed = 0;
+    d->Hh.assign((size_t) (d->lzMax + 1) * (d->lzMax + 1), 0.0);
+    DCALL(dalloc(d, &d->V, n * (d->lzMax + 1)));
+    DCALL(dalloc(d, &d->Hd, (size_t) (d->lzMax + 1) * (d->lzMax + 1)));
+    DCALL(dalloc(d, &d->lv, n));
+    DCALL(dalloc(d, &d->lw, n));
+    DCALL(dalloc(d, &d->lz1, n));
+    DCALL(dalloc(d, &d->lz2, n));
+    DCALL(dalloc(d, &d->lwarm, n));
+    DCALL(dalloc(d, &d->ly, 64));
+    DCALL(dalloc(d, &d->mv1, n));
+    DCALL(dalloc(d, &d->mv2, n));
+    DCALL(dalloc(d, &d->M, m * m));
+    DCALL(dalloc(d, &d->jac, m));
+    DCALL(dalloc(d, &d->cr, m));
+    DCALL(dalloc(d, &d->crn, m));
+    DCALL(dalloc(d, &d->cp, m));
+    DCALL(dalloc(d, &d->cz, m));
+    DCALL(dalloc(d, &d->cMp, m));
+    DCALL(dalloc(d, &d->cx, m));
+    DCALL(dalloc(d, &d->crhs, m));
+    DCALL(dalloc(d, &d->ctmp, m));
+    // CG defaults (linalg/hdsdp_linsolver.c:718-728)
+    d->absTol = 1e-06;
+    d->relTol = 1e-06;
+    d->useJacobi = d->paramUseJacobi = 1;
+    d->maxIter = std::max(50, nRow / 20);
+    d->nRestart = 20;
+    hdk::fill(d->st, d->M, (long) m * m, 0.0);
+    for (int k = 0; k < 3; ++k) hdk::fill(d->st, d->S[k], (long) nn, 0.0);
+    return check_launch();
+}
+
+hdsdp_retcode hdsdp_dev_load(hdsdp_dev *d, const hdsdp_coeff_view *cv) {
+    const int m = d->m, n = d->n;
+    const long npk = (long) n * (n + 1) / 2;
+    DevCoeffs &c = d->c;
+    c.m = m;
+    c.n = n;
+    std::vector<int> ctype(m + 1), cBeg(m + 2, 0), dSlot(m + 1, -1), rSlot(m + 1, -1);
+    std::vector<double> csign(m + 1, 0.0);
+    std::vector<int> cRow, cCol, denseCon, rCon;
+    std::vector<double> cVal;
+    d->cType = cv[0].type;
+    d->hasZero = 0;
+    for (int k = 0; k <= m; ++k) {
+        ctype[k] = cv[k].type;
+        csign[k] = cv[k].sign;
+        if (k > 0 && cv[k].type == HDSDP_COEFF_ZERO) d->hasZero = 1;
+        if (cv[k].type == HDSDP_COEFF_SPARSE) {
+            for (int e = 0; e < cv[k].nnz; ++e) {
+                cRow.push_back(cv[k].row[e]);
+                cCol.push_back(cv[k].col[e]);
+                cVal.push_back(cv[k].val[e]);
+            }
+        } else if (cv[k].type == HDSDP_COEFF_SPR1) {
+            for (int e = 0; e < cv[k].nnz; ++e) {
+                cRow.push_back(cv[k].row[e]);
+                cCol.push_back(cv[k].row[e]);
+                cVal.push_back(cv[k].val[e]);
+            }
+        }
+        cBeg[k + 1] = (int) cRow.size();
+    }
+    // dense coefficients: constraints in order, C last (assembly order of the reference)
+    for (int k = 1; k <= m; ++k)
+        if (ctype[k] == HDSDP_COEFF_DENSE) denseCon.push_back(k);
+    if (ctype[0] == HDSDP_COEFF_DENSE) denseCon.push_back(0);
+    for (size_t q = 0; q < denseCon.size(); ++q) dSlot[denseCon[q]] = (int) q;
+    for (int k = 1; k <= m; ++k)
+        if (ctype[k] == HDSDP_COEFF_DSR1) rCon.push_back(k);
+    for (size_t q = 0; q < rCon.size(); ++q) rSlot[rCon[q]] = (int) q;
+    c.cDSR1 = (ctype[0] == HDSDP_COEFF_DSR1);
+    if (c.cDSR1) rSlot[0] = (int) rCon.size();
+    c.nDense = (int) denseCon.size();
+    c.nR1 = (int) rCon.size();
+    c.cR1Sign = cv[0].sign;
+
+    // scatter map over SPARSE / SPR1 entries of the lower triangle, C last
+    struct Ent { long pos; int con; int order; double val; };
+    std::vector<Ent> ents;
+    auto add_entries = [&](int k, int order) {
+        if (ctype[k] == HDSDP_COEFF_SPARSE) {
+            for (int e = cBeg[k]; e < cBeg[k + 1]; ++e)
+                ents.push_back({cRow[e] + (long) cCol[e] * n, k, order, cVal[e]});
+        } else if (ctype[k] == HDSDP_COEFF_SPR1) {
+            for (int p = cBeg[k]; p < cBeg[k + 1]; ++p)
+                for (int q = cBeg[k]; q <= p; ++q)
+                    ents.push_back({cRow[p] + (long) cRow[q] * n, k, order, csign[k] * cVal[p] * cVal[q]});
+        }
+    };
+    for (int k = 1; k <= m; ++k) add_entries(k, k);
+    add_entries(0, m + 1);
+    std::stable_sort(ents.begin(), ents.end(), [](const Ent &a, const Ent &b) {
+        return a.pos != b.pos ? a.pos < b.pos : a.order < b.order;
+    });
+    std::vector<long> posIdx;
+    std::vector<int> posBeg, entCon;
+    std::vector<double> entVal;
+    for (size_t e = 0; e < ents.size(); ++e) {
+        if (e == 0 || ents[e].pos != ents[e - 1].pos) {
+            posIdx.push_back(ents[e].pos);
+            posBeg.push_back((int) e);
+        }
+        entCon.push_back(ents[e].con);
+        entVal.push_back(ents[e].val);
+    }
+    posBeg.push_back((int) ents.size());
+    c.nPos = (int) posIdx.size();
+
+    // small-constraint table for the pair kernel
+    const int K = hdk::SMALL_K;
+    std::vector<int> smKind(m + 1, hdk::SM_NONE), smNnz(m + 1, 0), smRow((m + 1) * K, 0), smCol((m + 1) * K, 0);
+    std::vector<double> smVal((m + 1) * K, 0.0);
+    d->heavy.clear();
+    for (int k = 1; k <= m; ++k) {
+        int t = ctype[k], nz = cBeg[k + 1] - cBeg[k];
+        if (t == HDSDP_COEFF_ZERO) continue;
+        if ((t == HDSDP_COEFF_SPARSE || t == HDSDP_COEFF_SPR1) && nz <= K) {
+            smKind[k] = (t == HDSDP_COEFF_SPARSE) ? hdk::SM_SP : hdk::SM_R1;
+            smNnz[k] = nz;
+            for (int e = 0; e < nz; ++e) {
+                smRow[k * K + e] = cRow[cBeg[k] + e];
+                smCol[k * K + e] = cCol[cBeg[k] + e];
+                smVal[k * K + e] = cVal[cBeg[k] + e];
+            }
+        } else {
+            smKind[k] = hdk::SM_HEAVY;
+            d->heavy.push_back(k);
+        }
+    }
+
+    auto up_i = [&](int **dst, const std::vector<int> &v) -> hdsdp_retcode {
+        DCALL(dalloc(d, dst, v.size()));
+        if (!v.empty()) HIP_OK(hipMemcpy(*dst, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice));
+        return HDSDP_RETCODE_OK;
+    };
+    auto up_d = [&](double **dst, const std::vector<double> &v) -> hdsdp_retcode {
+        DCALL(dalloc(d, dst, v.size()));
+        if (!v.empty()) HIP_OK(hipMemcpy(*dst, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+        return HDSDP_RETCODE_OK;
+    };
+    DCALL(up_i(&c.ctype, ctype));
+    DCALL(up_d(&c.csign, csign));
+    DCALL(up_i(&c.cBeg, cBeg));
+    DCALL(up_i(&c.cRow, cRow));
+    DCALL(up_i(&c.cCol, cCol));
+    DCALL(up_d(&c.cVal, cVal));
+    DCALL(up_i(&c.denseCon, denseCon));
+    DCALL(up_i(&c.dSlot, dSlot));
+    DCALL(up_i(&c.rCon, rCon));
+    DCALL(up_i(&c.rSlot, rSlot));
+    DCALL(dalloc(d, &c.dVal, (size_t) c.nDense * npk));
+    for (int q = 0; q < c.nDense; ++q)
+        HIP_OK(hipMemcpy(c.dVal + (size_t) q * npk, cv[denseCon[q]].val, npk * sizeof(double), hipMemcpyHostToDevice));
+    int nF = c.nR1 + (c.cDSR1 ? 1 : 0);
+    DCALL(dalloc(d, &c.rF, (size_t) n * nF));
+    for (int q = 0; q < c.nR1; ++q)
+        HIP_OK(hipMemcpy(c.rF + (size_t) q * n, cv[rCon[q]].val, n * sizeof(double), hipMemcpyHostToDevice));
+    if (c.cDSR1) HIP_OK(hipMemcpy(c.rF + (size_t) c.nR1 * n, cv[0].val, n * sizeof(double), hipMemcpyHostToDevice));
+    c.cR1F = c.rF + (size_t) c.nR1 * n;
+    DCALL(dalloc(d, &c.r1w, c.nR1));
+    {
+        DCALL(dalloc(d, &c.posIdx, posIdx.size()));
+        if (!posIdx.empty())
+            HIP_OK(hipMemcpy(c.posIdx, posIdx.data(), posIdx.size() * sizeof(long), hipMemcpyHostToDevice));
+    }
+    DCALL(up_i(&c.posBeg, posBeg));
+    DCALL(up_i(&c.entCon, entCon));
+    DCALL(up_d(&c.entVal, entVal));
+    DCALL(up_i(&c.smKind, smKind));
+    DCALL(up_i(&c.smNnz, smNnz));
+    DCALL(up_i(&c.smRow, smRow));
+    DCALL(up_i(&c.smCol, smCol));
+    DCALL(up_d(&c.smVal, smVal));
+    size_t workSize = std::max((size_t) n * n, (size_t) n * std::max(c.nR1, 1));
+    DCALL(dalloc(d, &d->work, workSize));
+    return check_launch();
+}
+
+hdsdp_retcode hdsdp_dev_scale_obj(hdsdp_dev *d, double s) {
+    // rescale C where it is stored (sparse entries in the CSR and the scatter map,
+    // dense packed block, or the rank-one sign)
+    if (s == 1.0) return HDSDP_RETCODE_OK;
+    DevCoeffs &c = d->c;
+    if (d->cType == HDSDP_COEFF_SPR1 || d->cType == HDSDP_COEFF_DSR1) {
+        double sg = 0.0;
+        HIP_OK(hipMemcpy(&sg, c.csign, sizeof(double), hipMemcpyDeviceToHost));
+        sg *= s;
+        HIP_OK(hipMemcpy(c.csign, &sg, sizeof(double), hipMemcpyHostToDevice));
+        c.cR1Sign *= s;
+        if (d->cType == HDSDP_COEFF_DSR1) return HDSDP_RETCODE_OK;
+    }
+    if (d->cType == HDSDP_COEFF_DENSE) {
+        long npk = (long) d->n * (d->n + 1) / 2;
+        std::vector<int> dSlot(1);
+        HIP_OK(hipMemcpy(dSlot.data(), c.dSlot, sizeof(int), hipMemcpyDeviceToHost));
+        std::vector<double> buf(npk);
+        double *p = c.dVal + (long) dSlot[0] * npk;
+        HIP_OK(hipMemcpy(buf.data(), p, npk * sizeof(double), hipMemcpyDeviceToHost));
+        for (auto &v : buf) v *= s;
+        HIP_OK(hipMemcpy(p, buf.data(), npk * sizeof(double), hipMemcpyHostToDevice));
+        return HDSDP_RETCODE_OK;
+    }
+    // SPARSE or SPR1 C: its entries are the last of each scatter group and the first CSR segment
+    std::vector<int> cBeg(2);
+    HIP_OK(hipMemcpy(cBeg.data(), c.cBeg, 2 * sizeof(int), hipMemcpyDeviceToHost));
+    if (d->cType == HDSDP_COEFF_SPARSE && cBeg[1] > 0) {
+        std::vector<double> v(cBeg[1]);
+        HIP_OK(hipMemcpy(v.data(), c.cVal, v.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (auto &x : v) x *= s;
+        HIP_OK(hipMemcpy(c.cVal, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (c.nPos > 0) {
+        std::vector<int> posBeg(c.nPos + 1);
+        HIP_OK(hipMemcpy(posBeg.data(), c.posBeg, posBeg.size() * sizeof(int), hipMemcpyDeviceToHost));
+        int nEnt = posBeg[c.nPos];
+        std::vector<int> con(nEnt);
+        std::vector<double> val(nEnt);
+        HIP_OK(hipMemcpy(con.data(), c.entCon, nEnt * sizeof(int), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(val.data(), c.entVal, nEnt * sizeof(double), hipMemcpyDeviceToHost));
+        for (int e = 0; e < nEnt; ++e)
+            if (con[e] == 0) val[e] *= s;
+        HIP_OK(hipMemcpy(c.entVal, val.data(), nEnt * sizeof(double), hipMemcpyHostToDevice));
+    }
+    return HDSDP_RETCODE_OK;
+}
+
+void hdsdp_dev_destroy(hdsdp_dev **pdev) {
+    if (!pdev || !*pdev) return;
+    hdsdp_dev *d = *pdev;
+    if (d->st) hipStreamSynchronize(d->st);
+    for (void *p : d->allocs) hipFree(p);
+    double *hp[] = {d->hcoef, d->hvecA, d->hvecB, d->hsc, d->hvec_n, d->hY};
+    for (double *p : hp)
+        if (p) hipHostFree(p);
+    if (d->hinfo) hipHostFree(d->hinfo);
+    if (d->st) hipStreamDestroy(d->st);
+    delete d;
+    *pdev = nullptr;
+}
+
+hdsdp_retcode hdsdp_dev_stats(hdsdp_dev *d, double *stats, int nStats) {
+    for (int k = 0; k < nStats && k < 6; ++k) stats[k] = d->stats[k];
+    return HDSDP_RETCODE_OK;
+}
+
+// ---- dual buffers -----------------------------------------------------------------
+
+hdsdp_retcode hdsdp_dev_update_buffer(hdsdp_dev *d, double cCoef, double aScal, const double *aCoef,
+                                      double dEyeCoef, int which) {
+    const int m = d->m;
+    if (d->stagingBusy) DCALL(sync(d));
+    d->hcoef[0] = cCoef;
+    for (int i = 0; i < m; ++i) d->hcoef[i + 1] = aScal * aCoef[i];
+    HIP_OK(hipMemcpyAsync(d->coef_d, d->hcoef, sizeof(double) * (m + 1), hipMemcpyHostToDevice, d->st));
+    d->stagingBusy = true;
+    hdk::assemble_buffer(d->st, d->c, d->coef_d, cCoef, d->S[which], dEyeCoef, d->work);
+    return check_launch();
+}
+
+hdsdp_retcode hdsdp_dev_factorize(hdsdp_dev *d, int which, int *isPsd) {
+    const int n = d->n, f = fidx(which);
+    HIP_OK(hipMemcpyAsync(d->L[f], d->S[f], sizeof(double) * (size_t) n * n, hipMemcpyDeviceToDevice, d->st));
+    hdk::potrf<double>(d->st, n, d->L[f], n, d->T1, d->info_d);
+    HIP_OK(hipMemcpyAsync(d->hinfo, d->info_d, sizeof(int), hipMemcpyDeviceToHost, d->st));
+    DCALL(sync(d));
+    d->facVer[f] += 1;
+    d->stats[1] += 1;
+    *isPsd = (*d->hinfo == 0);
+    return check_launch();
+}
+
+hdsdp_retcode hdsdp_dev_axpy_check(hdsdp_dev *d, double dStep, int which, int *isPsd) {
+    const int f = fidx(which);
+    long nn = (long) d->n * d->n;
+    hdk::axpy_buffer(d->st, nn, dStep, d->S[0], d->S[2], d->S[f]);
+    return hdsdp_dev_factorize(d, f, isPsd);
+}
+
+hdsdp_retcode hdsdp_dev_logdet(hdsdp_dev *d, int which, double *logdet) {
+    const int f = fidx(which);
+    hdk::logdet_diag(d->st, d->n, d->L[f], d->n, d->sc_d + hdk::SC_LOGDET);
+    DCALL(read_scalars(d));
+    *logdet = d->hsc[hdk::SC_LOGDET];
+    return HDSDP_RETCODE_OK;
+}
+
+hdsdp_retcode hdsdp_dev_ratio_test(hdsdp_dev *d, int which, double *maxStep) {
+    const int f = fidx(which);
+    if (d->n == 1) {
+        double s = 0.0, ds = 0.0;
+        HIP_OK(hipMemcpyAsync(d->hsc, d->S[f], sizeof(double), hipMemcpyDeviceToHost, d->st));
+        HIP_OK(hipMemcpyAsync(d->hsc + 1, d->S[2], sizeof(double), hipMemcpyDeviceToHost, d->st));
+        DCALL(sync(d));
+        s = d->hsc[0];
+        ds = d->hsc[1];
+        *maxStep = (ds > 0.0) ? HDSDP_INFINITY : (-s / ds);
+        return HDSDP_RETCODE_OK;
+    }
+    DCALL(ensure_linv(d, f));
+    hdk::mirror_lower<double>(d->st, d->n, d->S[2], d->n);
+    return lanczos(d, f, maxStep);
+}
+
+// ---- Schur complement ---------------------------------------------------------------
+
+hdsdp_retcode hdsdp_dev_schur(hdsdp_dev *d, int typeKKT, double Rd, double *asinv, double *asinvrd,
+                              double *traceSinv) {
+    const int m = d->m, n = d->n;
+    if (typeKKT == 2 || typeKKT == 3) return HDSDP_RETCODE_FAILED;
+    DCALL(ensure_linv(d, 0));
+    // W = S^-1 = L^-T L^-1
+    hdk::gemm<double>(d->st, true, false, false, n, n, n, 1.0, d->Li[0], n, 0, d->Li[0], n, 0, 0.0, d->W, n, 0, 1);
+    if (Rd != 0.0)
+        hdk::gemm<double>(d->st, false, false, false, n, n, n, 1.0, d->W, n, 0, d->W, n, 0, 0.0, d->W2, n, 0, 1);
+    hdk::traces(d->st, d->c, 1, m, d->W, Rd != 0.0 ? d->W2 : nullptr, Rd, d->vecA_d, d->vecB_d);
+    *traceSinv = 0.0;
+    if (typeKKT != 1 && Rd != 0.0) hdk::trace_diag(d->st, n, d->W, d->sc_d + hdk::SC_AUX);
+    if (typeKKT != 1) {
+        if (d->hasZero) hdk::fill(d->st, d->M, (long) m * m, 0.0);
+        hdk::schur_pairs(d->st, d->c, d->W, d->M, m);
+        for (int h : d->heavy) {
+            // B_h = W A_h W, then M(h, k) = <A_k, B_h>
+            hdk::densify(d->st, d->c, h, d->T1);
+            hdk::gemm<double>(d->st, false, false, false, n, n, n, 1.0, d->W, n, 0, d->T1, n, 0, 0.0, d->T2, n, 0, 1);
+            hdk::gemm<double>(d->st, false, false, false, n, n, n, 1.0, d->T2, n, 0, d->W, n, 0, 0.0, d->T1, n, 0, 1);
+            hdk::traces(d->st, d->c, 1, m, d->T1, nullptr, 0.0, d->ctmp, nullptr);
+            hdk::scatter_row(d->st, m, h - 1, d->ctmp, d->M, m);
+        }
+        d->stats[0] += 1;
+    }
+    DCALL(check_launch());
+    DCALL(download(d, asinv, d->vecA_d, m, d->hvecA));
+    if (Rd != 0.0) {
+        DCALL(download(d, asinvrd, d->vecB_d, m, d->hvecB));
+    } else {
+        memset(asinvrd, 0, sizeof(double) * m);
+    }
+    if (typeKKT != 1 && Rd != 0.0) {
+        DCALL(read_scalars(d));
+        *traceSinv = d->hsc[hdk::SC_AUX];
+    }
+    return HDSDP_RETCODE_OK;
+}
+
+// ---- Schur solve ----------------------------------------------------------------------
+
+hdsdp_retcode hdsdp_dev_kkt_params(hdsdp_dev *d, double relTol, double absTol, int maxIter, int nRestart) {
+    if (absTol > 0) d->absTol = absTol;
+    if (relTol > 0) d->relTol = relTol;
+    if (maxIter > 0) d->maxIter = maxIter;
+    if (nRestart > 0) d->nRestart = nRestart;
+    return HDSDP_RETCODE_OK;
+}
+
+hdsdp_retcode hdsdp_dev_kkt_add_diag(hdsdp_dev *d, const double *diag) {
+    DCALL(upload(d, d->ctmp, diag, d->m, d->hvecA));
+    hdk::add_diag(d->st, d->m, d->M, d->m, d->ctmp, 0.0);
+    return check_launch();
+}
+
+hdsdp_retcode hdsdp_dev_kkt_regularize(hdsdp_dev *d, double reg) {
+    hdk::regularize(d->st, d->m, d->M, d->m, reg);
+    return check_launch();
+}
+
+hdsdp_retcode hdsdp_dev_kkt_factorize(hdsdp_dev *d) {
+    hdk::mirror_lower<double>(d->st, d->m, d->M, d->m);
+    return build_precond(d);
+}
+
+hdsdp_retcode hdsdp_dev_kkt_solve(hdsdp_dev *d, const double *rhs, double *sol) {
+    DCALL(upload(d, d->crhs, rhs, d->m, d->hvecB));
+    int rc = cg_solve(d);
+    if (rc < 0) return HDSDP_RETCODE_FAILED;
+    if (rc == 1) return HDSDP_RETCODE_OK;  // tiny right-hand side: solution left as it was
+    return download(d, sol, d->cx, d->m, d->hvecB);
+}
+
+hdsdp_retcode hdsdp_dev_get_schur(hdsdp_dev *d, double *M) {
+    const int m = d->m;
+    hdk::mirror_lower<double>(d->st, m, d->M, m);
+    DCALL(sync(d));
+    HIP_OK(hipMemcpy(M, d->M, sizeof(double) * (size_t) m * m, hipMemcpyDeviceToHost));
+    return HDSDP_RETCODE_OK;
+}
+
+// ---- primal recovery ----------------------------------------------------------------------
+
+hdsdp_retcode hdsdp_dev_primal(hdsdp_dev *d, double mu, const double *y, const double *dy, int *ok) {
+    (void) y;
+    const int n = d->n;
+    DCALL(hdsdp_dev_factorize(d, 1, ok));
+    if (!*ok) return HDSDP_RETCODE_OK;
+    DCALL(hdsdp_dev_update_buffer(d, 0.0, 1.0, dy, 0.0, 2));
+    hdk::mirror_lower<double>(d->st, n, d->S[2], n);
+    DCALL(ensure_linv(d, 1));
+    // X = mu L^-T (I + L^-1 dS L^-T) L^-1 for the checker factor L
+    hdk::gemm<double>(d->st, false, false, false, n, n, n, 1.0, d->Li[1], n, 0, d->S[2], n, 0, 0.0, d->T1, n, 0, 1);
+    hdk::gemm<double>(d->st, false, true, false, n, n, n, 1.0, d->T1, n, 0, d->Li[1], n, 0, 0.0, d->T2, n, 0, 1);
+    hdk::sym_avg(d->st, n, d->T2, 1.0, 1.0, d->T1);
+    hdk::gemm<double>(d->st, true, false, false, n, n, n, 1.0, d->Li[1], n, 0, d->T1, n, 0, 0.0, d->T2, n, 0, 1);
+    hdk::gemm<double>(d->st, false, false, false, n, n, n, 1.0, d->T2, n, 0, d->Li[1], n, 0, 0.0, d->T1, n, 0, 1);
+    hdk::sym_avg(d->st, n, d->T1, mu, 0.0, d->X);
+    return check_launch();
+}
+
+hdsdp_retcode hdsdp_dev_primal_stats(hdsdp_dev *d, double *AX, double *traceCX, double *XdotS, double *traceX) {
+    const int m = d->m;
+    hdk::traces(d->st, d->c, 0, m + 1, d->X, nullptr, 0.0, d->vecA_d, nullptr);
+    hdk::xdots(d->st, d->n, d->X, d->S[0], d->sc_d + hdk::SC_AUX);
+    DCALL(check_launch());
+    DCALL(download(d, d->hvecB, d->vecA_d, m + 1, d->hvecA));
+    *traceCX = d->hvecB[0];
+    memcpy(AX, d->hvecB + 1, sizeof(double) * m);
+    DCALL(read_scalars(d));
+    *XdotS = d->hsc[hdk::SC_AUX];
+    *traceX = d->hsc[hdk::SC_AUX + 1];
+    return HDSDP_RETCODE_OK;
+}
+
+hdsdp_retcode hdsdp_dev_get_primal(hdsdp_dev *d, double *X) {
+    HIP_OK(hipMemcpy(X, d->X, sizeof(double) * (size_t) d->n * d->n, hipMemcpyDeviceToHost));
+    return HDSDP_RETCODE_OK;
+}
+
+hdsdp_retcode hdsdp_dev_get_dual(hdsdp_dev *d, double *S) {
+    const int n = d->n;
+    HIP_OK(hipMemcpyAsync(d->T1, d->S[0], sizeof(double) * (size_t) n * n, hipMemcpyDeviceToDevice, d->st));
+    hdk::mirror_lower<double>(d->st, n, d->T1, n);
+    DCALL(sync(d));
+    HIP_OK(hipMemcpy(S, d->T1, sizeof(double) * (size_t) n * n, hipMemcpyDeviceToHost));
+    return HDSDP_RETCODE_OK;
+}
+
+// ---- adjoint -------------------------------------------------------------------------------
+
+hdsdp_retcode hdsdp_dev_adjoint(hdsdp_dev *d, double mu, const double *dLdy, double *dLdb, double *dLdC) {
+    const int m = d->m, n = d->n;
+    std::vector<double> lam(m, 0.0);
+    DCALL(hdsdp_dev_kkt_solve(d, dLdy, lam.data()));
+    for (int i = 0; i < m; ++i) dLdb[i] = lam[i] / mu;
+    // dLdC = -W (sum_i lambda_i A_i) W with W = S^-1 of the dual factor
+    DCALL(ensure_linv(d, 0));
+    hdk::gemm<double>(d->st, true, false, false, n, n, n, 1.0, d->Li[0], n, 0, d->Li[0], n, 0, 0.0, d->W, n, 0, 1);
+    DCALL(hdsdp_dev_update_buffer(d, 0.0, 1.0, lam.data(), 0.0, 2));
+    hdk::mirror_lower<double>(d->st, n, d->S[2], n);
+    hdk::gemm<double>(d->st, false, false, false, n, n, n, 1.0, d->W, n, 0, d->S[2], n, 0, 0.0, d->T1, n, 0, 1);
+    hdk::gemm<double>(d->st, false, false, false, n, n, n, -1.0, d->T1, n, 0, d->W, n, 0, 0.0, d->T2, n, 0, 1);
+    DCALL(sync(d));
+    HIP_OK(hipMemcpy(dLdC, d->T2, sizeof(double) * (size_t) n * n, hipMemcpyDeviceToHost));
+    return HDSDP_RETCODE_OK;
+}
+
+}  // extern "C"

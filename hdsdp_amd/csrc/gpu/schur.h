@@ -1,0 +1,58 @@
+// schur.h -- device coefficient layout, buffer assembly and Schur kernels.
+#pragma once
+
+#include "dev_common.h"
+
+namespace hdk {
+
+constexpr int COEFF_ZERO = 0, COEFF_SPARSE = 1, COEFF_DENSE = 2, COEFF_SPR1 = 3, COEFF_DSR1 = 4;
+constexpr int SMALL_K = 4;                                   // entries per constraint in the pair kernel
+constexpr int SM_NONE = 0, SM_SP = 1, SM_R1 = 2, SM_HEAVY = 3;  // pair-kernel kinds
+
+// Device-resident coefficients of one SDP block (k = 0 is C, k = i + 1 is A_i).
+struct DevCoeffs {
+    int m, n;
+    int *ctype;       // m + 1
+    double *csign;    // m + 1 (rank-one signs)
+    int *cBeg;        // m + 2, CSR over SPARSE / SPR1 entries
+    int *cRow, *cCol;
+    double *cVal;
+    int nDense;       // DENSE coefficients in assembly order (C last)
+    int *denseCon;    // coefficient index per dense slot
+    int *dSlot;       // m + 1, slot or -1
+    double *dVal;     // nDense x n(n+1)/2
+    int nR1;          // DSR1 constraints (C excluded)
+    int *rCon;        // coefficient index per DSR1 slot
+    int *rSlot;       // m + 1, slot or -1 (C included when DSR1: slot nR1)
+    double *rF;       // n x (nR1 + [C is DSR1])
+    double *r1w;      // nR1 weights scratch
+    int cDSR1;        // C is a dense rank-one matrix
+    double cR1Sign;
+    double *cR1F;     // C factor (alias into rF)
+    int nPos;         // scatter map over SPARSE / SPR1 entries
+    long *posIdx;
+    int *posBeg;
+    int *entCon;
+    double *entVal;
+    int *smKind, *smNnz, *smRow, *smCol;  // (m + 1) x SMALL_K
+    double *smVal;
+};
+
+// target = sum_k coef[k] A_k (coef[0] multiplies C) + eye I on the lower triangle.
+// cCoef is coef[0] on the host (needed for a dense rank-one C).  work: n * nR1.
+void assemble_buffer(hipStream_t s, const DevCoeffs &c, const double *coef, double cCoef, double *target,
+                     double eye, double *work);
+void axpy_buffer(hipStream_t s, long nn, double step, const double *src, const double *dS, double *dst);
+
+// out1[k - k0] = <A_k, B1>, out2[k - k0] = s2 <A_k, B2> for k in [k0, k0 + cnt)
+void traces(hipStream_t s, const DevCoeffs &c, int k0, int cnt, const double *B1, const double *B2, double s2,
+            double *out1, double *out2);
+// M_ij = <A_i, W A_j W> for pairs of small constraints (lower triangle)
+void schur_pairs(hipStream_t s, const DevCoeffs &c, const double *W, double *M, long ldm);
+void scatter_row(hipStream_t s, int m, int h, const double *t, double *M, long ldm);
+void densify(hipStream_t s, const DevCoeffs &c, int k, double *out);
+void sym_avg(hipStream_t s, int n, const double *A, double scale, double diag, double *out);
+void xdots(hipStream_t s, int n, const double *X, const double *S, double *out);
+void trace_diag(hipStream_t s, int n, const double *A, double *out);
+
+}  // namespace hdk

@@ -1,0 +1,394 @@
+// schur.hip -- dual-buffer assembly and Schur-complement kernels.
+//
+// Coefficient layout on the device (built once at load):
+//   ctype[k]            kind of coefficient k (0 = C, 1..m = A_k-1)
+//   cBeg/cRow/cCol/cVal CSR over coefficients for SPARSE (lower triplets) and
+//                       SPR1 (cRow = factor index, cVal = factor value)
+//   dSlot/dVal          DENSE packed lower columns, one n(n+1)/2 block each
+//   rSlot/rF            DSR1 dense factors (n x nR1), sign in csign
+//   scatter map         every SPARSE/SPR1 entry of the lower triangle grouped by
+//                       position, constraints in order and C last, so buffer
+//                       assembly is a deterministic gather with the reference's
+//                       summation order (interface/hdsdp_conic_sdp.c:373-379).
+//   small table         up to SMALL_K entries per constraint for the pair kernel.
+#include "schur.h"
+
+namespace hdk {
+
+// ---- buffer assembly -----------------------------------------------------------
+
+// target(i, j), i >= j: sum over dense coefficients of coef * packed value; zero elsewhere.
+__global__ void assemble_dense_kernel(int n, int nDense, const int *denseCon, const double *dVal, const double *coef,
+                                      double *target) {
+    const int j = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v = 0.0;
+    if (i >= j) {
+        long p = (long) j * n - (long) j * (j - 1) / 2 + (i - j);
+        long npk = (long) n * (n + 1) / 2;
+        for (int d = 0; d < nDense; ++d) {
+            double c = coef[denseCon[d]];
+            if (c != 0.0) v += c * dVal[(long) d * npk + p];
+        }
+    }
+    target[i + (long) j * n] = v;
+}
+
+__global__ void scatter_kernel(int nPos, const long *posIdx, const int *posBeg, const int *entCon,
+                               const double *entVal, const double *coef, double *target) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nPos) return;
+    double v = target[posIdx[q]];
+    for (int e = posBeg[q]; e < posBeg[q + 1]; ++e) {
+        double c = coef[entCon[e]];
+        if (c != 0.0) v += c * entVal[e];
+    }
+    target[posIdx[q]] = v;
+}
+
+// target(i, j) += w * f_i f_j, i >= j (one dense rank-one coefficient)
+__global__ void rank1_update_kernel(int n, double w, const double *f, double *target) {
+    const int j = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || i < j) return;
+    target[i + (long) j * n] += w * f[i] * f[j];
+}
+
+// Fw[:, k] = w_k F[:, k]
+__global__ void scale_cols_kernel(int n, int k, const double *F, const double *w, double *Fw) {
+    long total = (long) n * k;
+    for (long e = blockIdx.x * (long) blockDim.x + threadIdx.x; e < total; e += (long) gridDim.x * blockDim.x) {
+        Fw[e] = F[e] * w[e / n];
+    }
+}
+
+__global__ void dsr1_weights_kernel(int nR1, const int *rCon, const double *csign, const double *coef, double *w) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nR1) w[q] = coef[rCon[q]] * csign[rCon[q]];
+}
+
+void assemble_buffer(hipStream_t s, const DevCoeffs &c, const double *coef, double cCoef, double *target,
+                     double eye, double *work) {
+    const int n = c.n;
+    dim3 g((n + 255) / 256, n);
+    assemble_dense_kernel<<<g, 256, 0, s>>>(n, c.nDense, c.denseCon, c.dVal, coef, target);
+    if (c.nPos > 0) scatter_kernel<<<(c.nPos + 255) / 256, 256, 0, s>>>(c.nPos, c.posIdx, c.posBeg, c.entCon,
+                                                                         c.entVal, coef, target);
+    if (c.nR1 > 0) {
+        // target += F diag(coef * sign) F^T (lower) as one GEMM
+        dsr1_weights_kernel<<<(c.nR1 + 255) / 256, 256, 0, s>>>(c.nR1, c.rCon, c.csign, coef, c.r1w);
+        scale_cols_kernel<<<4096, 256, 0, s>>>(n, c.nR1, c.rF, c.r1w, work);
+        gemm<double>(s, false, true, true, n, n, c.nR1, 1.0, work, n, 0, c.rF, n, 0, 1.0, target, n, 0, 1);
+    }
+    if (c.cDSR1 && cCoef != 0.0) {
+        // C = sign f f^T enters last among the coefficients, as a rank-one update
+        rank1_update_kernel<<<g, 256, 0, s>>>(n, cCoef * c.cR1Sign, c.cR1F, target);
+    }
+    if (eye != 0.0) add_diag(s, n, target, n, nullptr, eye);
+}
+
+__global__ void axpy_buffer_kernel(long nn, double step, const double *src, const double *dS, double *dst) {
+    for (long e = blockIdx.x * (long) blockDim.x + threadIdx.x; e < nn; e += (long) gridDim.x * blockDim.x)
+        dst[e] = src[e] + step * dS[e];
+}
+
+void axpy_buffer(hipStream_t s, long nn, double step, const double *src, const double *dS, double *dst) {
+    long blocks = (nn + 255) / 256;
+    axpy_buffer_kernel<<<blocks > 8192 ? 8192 : blocks, 256, 0, s>>>(nn, step, src, dS, dst);
+}
+
+// ---- traces <A_k, B> -----------------------------------------------------------------
+
+__device__ inline double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    return v;
+}
+
+// out1[k] = <A_k, B1>, out2[k] = s2 <A_k, B2> (B2 may be NULL) for coefficients k0 .. k0 + cnt - 1,
+// written to out[k - k0].  One workgroup of 256 threads per coefficient.
+__global__ __launch_bounds__(256) void trace_kernel(int n, DevCoeffs c, int k0, const double *B1, const double *B2,
+                                                    double s2, double *out1, double *out2) {
+    __shared__ double red1[4], red2[4];
+    const int k = k0 + blockIdx.x;
+    const int tid = threadIdx.x;
+    const int t = c.ctype[k];
+    double a1 = 0.0, a2 = 0.0;
+    if (t == COEFF_SPARSE) {
+        for (int e = c.cBeg[k] + tid; e < c.cBeg[k + 1]; e += 256) {
+            int r = c.cRow[e], cc = c.cCol[e];
+            double w = (r == cc) ? c.cVal[e] : 2.0 * c.cVal[e];
+            long idx = r + (long) cc * n;
+            a1 += w * B1[idx];
+            if (B2) a2 += w * B2[idx];
+        }
+    } else if (t == COEFF_SPR1) {
+        int b = c.cBeg[k], nnz = c.cBeg[k + 1] - b;
+        for (int e = tid; e < nnz * nnz; e += 256) {
+            int p = e % nnz, q = e / nnz;
+            double w = c.cVal[b + p] * c.cVal[b + q];
+            long idx = c.cRow[b + p] + (long) c.cRow[b + q] * n;
+            a1 += w * B1[idx];
+            if (B2) a2 += w * B2[idx];
+        }
+        a1 *= c.csign[k];
+        a2 *= c.csign[k];
+    } else if (t == COEFF_DENSE) {
+        const double *A = c.dVal + (long) c.dSlot[k] * ((long) n * (n + 1) / 2);
+        long colStart = 0;
+        for (int j = 0; j < n; ++j) {
+            for (int i = j + tid; i < n; i += 256) {
+                double av = A[colStart + (i - j)];
+                double w = (i == j) ? av : 2.0 * av;
+                long idx = i + (long) j * n;
+                a1 += w * B1[idx];
+                if (B2) a2 += w * B2[idx];
+            }
+            colStart += n - j;
+        }
+    } else if (t == COEFF_DSR1) {
+        // f^T B f with B symmetric: column i of B is read contiguously
+        const double *f = c.rF + (long) c.rSlot[k] * n;
+        for (int i = tid; i < n; i += 256) {
+            double r1 = 0.0, r2 = 0.0;
+            const double *b1 = B1 + (long) i * n;
+            const double *b2 = B2 ? B2 + (long) i * n : nullptr;
+            for (int j = 0; j < n; ++j) {
+                r1 += b1[j] * f[j];
+                if (b2) r2 += b2[j] * f[j];
+            }
+            a1 += f[i] * r1;
+            a2 += f[i] * r2;
+        }
+        a1 *= c.csign[k];
+        a2 *= c.csign[k];
+    }
+    a1 = wave_sum(a1);
+    a2 = wave_sum(a2);
+    if ((tid & 63) == 0) {
+        red1[tid >> 6] = a1;
+        red2[tid >> 6] = a2;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        out1[blockIdx.x] = red1[0] + red1[1] + red1[2] + red1[3];
+        if (out2) out2[blockIdx.x] = s2 * (red2[0] + red2[1] + red2[2] + red2[3]);
+    }
+}
+
+void traces(hipStream_t s, const DevCoeffs &c, int k0, int cnt, const double *B1, const double *B2, double s2,
+            double *out1, double *out2) {
+    if (cnt <= 0) return;
+    trace_kernel<<<cnt, 256, 0, s>>>(c.n, c, k0, B1, B2, s2, out1, out2);
+}
+
+// ---- Schur pair kernel over small sparse / rank-one constraints -----------------------
+
+struct SmallEnt {
+    int kind, nnz;
+    int r[SMALL_K], c[SMALL_K];
+    double v[SMALL_K];
+    double sign;
+};
+
+__device__ inline double pair_value(const SmallEnt &A, const SmallEnt &B, const double *W, int n) {
+#define WW(i, j) W[(i) + (long) (j) * n]
+    if (A.kind == SM_R1 && B.kind == SM_R1) {
+        double q = 0.0;
+        for (int p = 0; p < A.nnz; ++p)
+            for (int t = 0; t < B.nnz; ++t) q += A.v[p] * B.v[t] * WW(A.r[p], B.r[t]);
+        return q * q * A.sign * B.sign;
+    }
+    const SmallEnt &S = (A.kind == SM_SP) ? A : B;   // sparse side
+    const SmallEnt &O = (A.kind == SM_SP) ? B : A;   // other side
+    double res = 0.0;
+    for (int a = 0; a < S.nnz; ++a) {
+        int ar = S.r[a], ac = S.c[a];
+        double tb = 0.0;
+        if (O.kind == SM_SP) {
+            for (int b = 0; b < O.nnz; ++b) {
+                int br = O.r[b], bc = O.c[b];
+                tb += O.v[b] * WW(ar, br) * WW(ac, bc);
+                if (br != bc) tb += O.v[b] * WW(ar, bc) * WW(ac, br);
+            }
+        } else {
+            double ur = 0.0, uc = 0.0;
+            for (int p = 0; p < O.nnz; ++p) {
+                ur += O.v[p] * WW(ar, O.r[p]);
+                uc += O.v[p] * WW(ac, O.r[p]);
+            }
+            tb = O.sign * ur * uc;
+        }
+        res += (ar == ac) ? 0.5 * S.v[a] * tb : S.v[a] * tb;
+    }
+    return 2.0 * res;
+#undef WW
+}
+
+__device__ inline void load_small(const DevCoeffs &c, int k, SmallEnt &e) {
+    e.kind = c.smKind[k];
+    e.nnz = c.smNnz[k];
+    e.sign = c.csign[k];
+    for (int q = 0; q < SMALL_K; ++q) {
+        e.r[q] = c.smRow[(long) k * SMALL_K + q];
+        e.c[q] = c.smCol[(long) k * SMALL_K + q];
+        e.v[q] = c.smVal[(long) k * SMALL_K + q];
+    }
+}
+
+// One 64 x 64 lower tile of M per workgroup; thread (il, jg) computes rows i0 + il
+// against columns j0 + jg, jg + 4, ...  Constraint k of M is coefficient k + 1.
+__global__ __launch_bounds__(256) void schur_pair_kernel(int m, int n, DevCoeffs c, const double *W, double *M,
+                                                         long ldm) {
+    __shared__ SmallEnt Ti[64];
+    __shared__ SmallEnt Tj[64];
+    long t = blockIdx.x;
+    long bi = (long) ((sqrt(8.0 * (double) t + 1.0) - 1.0) / 2.0);
+    while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
+    while (bi * (bi + 1) / 2 > t) --bi;
+    long bj = t - bi * (bi + 1) / 2;
+    const int i0 = (int) bi * 64, j0 = (int) bj * 64;
+    const int tid = threadIdx.x;
+    if (tid < 64) {
+        int i = i0 + tid;
+        if (i < m) load_small(c, i + 1, Ti[tid]);
+        else Ti[tid].kind = SM_NONE;
+    } else if (tid < 128) {
+        int j = j0 + tid - 64;
+        if (j < m) load_small(c, j + 1, Tj[tid - 64]);
+        else Tj[tid - 64].kind = SM_NONE;
+    }
+    __syncthreads();
+    const int il = tid & 63, jg = tid >> 6;
+    const int i = i0 + il;
+    if (i >= m) return;
+    const SmallEnt &A = Ti[il];
+    if (A.kind != SM_SP && A.kind != SM_R1) return;
+    for (int jl = jg; jl < 64; jl += 4) {
+        int j = j0 + jl;
+        if (j >= m || j > i) continue;
+        const SmallEnt &B = Tj[jl];
+        if (B.kind != SM_SP && B.kind != SM_R1) continue;
+        M[i + (long) j * ldm] = pair_value(A, B, W, n);
+    }
+}
+
+void schur_pairs(hipStream_t s, const DevCoeffs &c, const double *W, double *M, long ldm) {
+    const int m = c.m;
+    long nt = (m + 63) / 64;
+    long tiles = nt * (nt + 1) / 2;
+    schur_pair_kernel<<<tiles, 256, 0, s>>>(m, c.n, c, W, M, ldm);
+}
+
+// M(max(h, k), min(h, k)) = t[k] for all k (row/column h of M from a trace vector)
+__global__ void scatter_row_kernel(int m, int h, const double *t, double *M, long ldm) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    int i = (k > h) ? k : h, j = (k > h) ? h : k;
+    M[i + (long) j * ldm] = t[k];
+}
+
+void scatter_row(hipStream_t s, int m, int h, const double *t, double *M, long ldm) {
+    scatter_row_kernel<<<(m + 255) / 256, 256, 0, s>>>(m, h, t, M, ldm);
+}
+
+// dense full symmetric copy of coefficient k into out (n x n)
+__global__ void densify_kernel(int n, DevCoeffs c, int k, double *out) {
+    int t = c.ctype[k];
+    if (t == COEFF_SPARSE || t == COEFF_SPR1) {
+        int b = c.cBeg[k], e1 = c.cBeg[k + 1];
+        if (t == COEFF_SPARSE) {
+            for (int e = b + threadIdx.x + blockIdx.x * blockDim.x; e < e1; e += blockDim.x * gridDim.x) {
+                int r = c.cRow[e], cc = c.cCol[e];
+                out[r + (long) cc * n] = c.cVal[e];
+                out[cc + (long) r * n] = c.cVal[e];
+            }
+        } else {
+            int nnz = e1 - b;
+            for (int e = threadIdx.x + blockIdx.x * blockDim.x; e < nnz * nnz; e += blockDim.x * gridDim.x) {
+                int p = e % nnz, q = e / nnz;
+                out[c.cRow[b + p] + (long) c.cRow[b + q] * n] = c.csign[k] * c.cVal[b + p] * c.cVal[b + q];
+            }
+        }
+    }
+}
+
+void densify(hipStream_t s, const DevCoeffs &c, int k, double *out) {
+    fill(s, out, (long) c.n * c.n, 0.0);
+    densify_kernel<<<64, 256, 0, s>>>(c.n, c, k, out);
+}
+
+// ---- small helpers ------------------------------------------------------------------
+
+// out = s * (A + A^T) / 2 + d * I (A full n x n)
+__global__ void sym_avg_kernel(int n, const double *A, double s, double d, double *out) {
+    long total = (long) n * n;
+    for (long e = blockIdx.x * (long) blockDim.x + threadIdx.x; e < total; e += (long) gridDim.x * blockDim.x) {
+        long i = e % n, j = e / n;
+        double v = 0.5 * (A[i + j * n] + A[j + i * n]);
+        if (i == j) v += d;
+        out[e] = s * v;
+    }
+}
+
+void sym_avg(hipStream_t s, int n, const double *A, double scale, double diag, double *out) {
+    sym_avg_kernel<<<4096, 256, 0, s>>>(n, A, scale, diag, out);
+}
+
+// out[0] = sum_{i >= j} (2 - [i == j]) X_ij S_ij ; out[1] = tr(X) ; out[2] = tr(W)-like trace of A (diag)
+__global__ __launch_bounds__(1024) void xdots_kernel(int n, const double *X, const double *S, double *out) {
+    __shared__ double red[17];
+    double a = 0.0, t = 0.0;
+    long total = (long) n * n;
+    for (long e = threadIdx.x; e < total; e += blockDim.x) {
+        long i = e % n, j = e / n;
+        if (i > j) a += 2.0 * X[e] * S[e];
+        else if (i == j) {
+            a += X[e] * S[e];
+            t += X[e];
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off);
+        t += __shfl_down(t, off);
+    }
+    __shared__ double red2[17];
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6] = a;
+        red2[threadIdx.x >> 6] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sa = 0.0, st = 0.0;
+        for (int w = 0; w < (int) (blockDim.x >> 6); ++w) {
+            sa += red[w];
+            st += red2[w];
+        }
+        out[0] = sa;
+        out[1] = st;
+    }
+}
+
+void xdots(hipStream_t s, int n, const double *X, const double *S, double *out) {
+    xdots_kernel<<<1, 1024, 0, s>>>(n, X, S, out);
+}
+
+__global__ __launch_bounds__(1024) void trace_diag_kernel(int n, const double *A, double *out) {
+    __shared__ double red[17];
+    double a = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) a += A[i + (long) i * n];
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < (int) (blockDim.x >> 6); ++w) s += red[w];
+        out[0] = s;
+    }
+}
+
+void trace_diag(hipStream_t s, int n, const double *A, double *out) {
+    trace_diag_kernel<<<1, 1024, 0, s>>>(n, A, out);
+}
+
+}  // namespace hdk

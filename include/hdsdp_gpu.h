@@ -122,6 +122,11 @@ hdsdp_retcode hdsdp_dev_kkt_factorize( hdsdp_dev *dev );
  * (conjGradSolve, linalg/hdsdp_linsolver.c:855-997).  rhs and sol may alias. */
 hdsdp_retcode hdsdp_dev_kkt_solve( hdsdp_dev *dev, const double *rhs, double *sol );
 
+/* Copy the Schur matrix M (m x m, full symmetric, bound-cone diagonal and
+ * regularization included) to the host; the matrix the CG works on
+ * (interface/hdsdp_schur.c:293-326 exports only its vectors). */
+hdsdp_retcode hdsdp_dev_get_schur( hdsdp_dev *dev, double *M );
+
 /* ---- primal recovery and solution checks -------------------------------- */
 
 /* X = mu (S^-1 + S^-1 (sum dy_i A_i) S^-1) with S = C - A^T y (+ perturbation)

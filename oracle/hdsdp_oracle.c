@@ -739,6 +739,13 @@ hdsdp_retcode hdsdp_dev_kkt_solve( hdsdp_dev *d, const double *rhs, double *sol 
     return cg_solve(d, d->rhsbuf, sol);
 }
 
+hdsdp_retcode hdsdp_dev_get_schur( hdsdp_dev *d, double *M ) {
+    int m = d->m;
+    for ( int j = 0; j < m; ++j )
+        for ( int i = j; i < m; ++i ) FE(M, m, i, j) = FE(M, m, j, i) = FE(d->M, m, i, j);
+    return HDSDP_RETCODE_OK;
+}
+
 /* --------------------------------------------------------- primal recovery */
 
 hdsdp_retcode hdsdp_dev_primal( hdsdp_dev *d, double mu, const double *y, const double *dy, int *ok ) {

@@ -1,0 +1,156 @@
+"""Device ABI parity: the HIP library against the CPU oracle and numpy closed
+forms on identical inputs (buffers, Cholesky/PSD check, log-det, Schur
+assembly, CG, Lanczos step length, primal recovery).  Tolerances: FP64 results
+within 1e-10 relative of the oracle; Lanczos step lengths within 1e-8."""
+import numpy as np
+import pytest
+
+import devtools as dt
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    # name: (n, m, kinds)
+    "spr1_unit": (37, 45, [dt.SPR1]),
+    "sparse_small": (41, 70, [dt.SPARSE]),
+    "mixed_sparse": (66, 90, [dt.SPARSE, dt.SPR1, "SPARSE_BIG", "SPR1_BIG"]),
+    "dense": (30, 40, [dt.DENSE]),
+    "dsr1": (35, 50, [dt.DSR1]),
+    "all_kinds_blocks": (130, 150, [dt.SPARSE, dt.SPR1, dt.DENSE, dt.DSR1, "SPARSE_BIG", dt.ZERO]),
+}
+
+
+def make_pair(gpu_lib, oracle, name, seed=0):
+    n, m, kinds = CASES[name]
+    rng = np.random.default_rng(seed)
+    coeffs = dt.rand_coeffs(rng, n, m, kinds)
+    return coeffs, dt.Dev(gpu_lib, coeffs), dt.Dev(oracle, coeffs), rng
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_buffer_factor_logdet(gpu_lib, oracle, name):
+    coeffs, g, o, rng = make_pair(gpu_lib, oracle, name)
+    y, eye = dt.interior_point(coeffs, rng)
+    for d in (g, o):
+        d.update(1.0, -1.0, y, eye, 0)
+        assert d.factorize(0) == 1
+    S = dt.dense_buffer(coeffs, 1.0, -1.0, y, eye)
+    ld = np.linalg.slogdet(S)[1]
+    assert abs(g.logdet(0) - ld) <= 1e-10 * max(1.0, abs(ld))
+    assert abs(g.logdet(0) - o.logdet(0)) <= 1e-11 * max(1.0, abs(ld))
+    # not positive definite: shift far below the smallest eigenvalue
+    lam = np.linalg.eigvalsh(S).min()
+    for d in (g, o):
+        d.update(1.0, -1.0, y, eye - lam - 1.0, 1)
+        assert d.factorize(1) == 0
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_schur_matches_oracle_and_closed_form(gpu_lib, oracle, name):
+    coeffs, g, o, rng = make_pair(gpu_lib, oracle, name, seed=1)
+    y, eye = dt.interior_point(coeffs, rng)
+    Rd = -0.37
+    res = []
+    for d in (g, o):
+        d.update(1.0, -1.0, y, eye, 0)
+        assert d.factorize(0) == 1
+        a, b, t = d.schur(0, Rd)
+        res.append((a, b, t, d.get_schur()))
+    (ga, gb, gt, gM), (oa, ob, ot, oM) = res
+    S = dt.dense_buffer(coeffs, 1.0, -1.0, y, eye)
+    W = np.linalg.inv(S)
+    A = [c.dense() for c in coeffs[1:]]
+    M = np.array([[np.trace(Ai @ W @ Aj @ W) for Aj in A] for Ai in A])
+    asinv = np.array([np.sum(Ai * W) for Ai in A])
+    asrd = Rd * np.array([np.sum(Ai * (W @ W)) for Ai in A])
+    assert rel(gM, M) < 1e-9
+    assert rel(gM, oM) < 1e-10
+    assert rel(ga, asinv) < 1e-9 and rel(ga, oa) < 1e-10
+    assert rel(gb, asrd) < 1e-9 and rel(gb, ob) < 1e-10
+    assert abs(gt - np.trace(W)) <= 1e-10 * abs(np.trace(W))
+    # corrector: vectors only, M untouched
+    for d in (g, o):
+        a2, b2, t2 = d.schur(1, Rd)
+        assert t2 == 0.0
+    assert rel(g.get_schur(), gM) == 0.0
+
+
+@pytest.mark.parametrize("name", ["spr1_unit", "mixed_sparse", "all_kinds_blocks"])
+def test_cg_solve(gpu_lib, oracle, name):
+    coeffs, g, o, rng = make_pair(gpu_lib, oracle, name, seed=2)
+    y, eye = dt.interior_point(coeffs, rng)
+    m = len(coeffs) - 1
+    diag = rng.uniform(0.1, 1.0, m)
+    rhs = rng.standard_normal(m)
+    sols = []
+    for d in (g, o):
+        d.update(1.0, -1.0, y, eye, 0)
+        assert d.factorize(0) == 1
+        d.schur(0, 0.0)
+        d.add_diag(diag)
+        d.regularize(1e-6)
+        d.kkt_params(5e-12, 1e-12)
+        assert d.kkt_factorize() == 0
+        sols.append(d.kkt_solve(rhs))
+    M = g.get_schur()
+    x = np.linalg.solve(M, rhs)
+    assert rel(sols[0], x) < 1e-7
+    assert rel(sols[0], sols[1]) < 1e-7
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_ratio_test_lanczos(gpu_lib, oracle, name):
+    coeffs, g, o, rng = make_pair(gpu_lib, oracle, name, seed=3)
+    y, eye = dt.interior_point(coeffs, rng)
+    dy = rng.standard_normal(len(coeffs) - 1)
+    steps = []
+    for d in (g, o):
+        d.update(1.0, -1.0, y, eye, 0)
+        assert d.factorize(0) == 1
+        d.update(0.0, -1.0, dy, 0.3, 2)
+        steps.append([d.ratio(0), d.ratio(0)])  # second call warm-starts
+    assert rel(steps[0], steps[1]) < 1e-8
+    S = dt.dense_buffer(coeffs, 1.0, -1.0, y, eye)
+    dS = dt.dense_buffer(coeffs, 0.0, -1.0, dy, 0.3)
+    L = np.linalg.cholesky(S)
+    Li = np.linalg.inv(L)
+    lam = np.linalg.eigvalsh(Li @ (-dS) @ Li.T).max()
+    if lam > 0:
+        # the safeguarded Lanczos bound never oversteps the boundary S + a dS = 0
+        assert steps[0][0] <= 1.0 / lam * (1 + 1e-8)
+        assert np.linalg.eigvalsh(S + 0.999 * steps[0][0] * dS).min() > 0
+
+
+@pytest.mark.parametrize("name", ["spr1_unit", "all_kinds_blocks"])
+def test_axpy_and_primal_recovery(gpu_lib, oracle, name):
+    coeffs, g, o, rng = make_pair(gpu_lib, oracle, name, seed=4)
+    y, eye = dt.interior_point(coeffs, rng)
+    m = len(coeffs) - 1
+    dy = 0.01 * rng.standard_normal(m)
+    out = []
+    for d in (g, o):
+        d.update(1.0, -1.0, y, eye, 0)
+        assert d.factorize(0) == 1
+        d.update(0.0, -1.0, dy, 0.0, 2)
+        ok = d.axpy_check(0.5, 1)
+        ld = d.logdet(1)
+        d.update(1.0, -1.0, y, eye, 1)  # checker = S for the recovery
+        assert d.primal(0.7, y, dy) == 1
+        out.append((ok, ld, d.get_primal(), d.primal_stats()))
+    (gok, gld, gX, gst), (ook, old, oX, ost) = out
+    assert gok == ook == 1
+    assert abs(gld - old) <= 1e-11 * max(1.0, abs(old))
+    S = dt.dense_buffer(coeffs, 1.0, -1.0, y, eye)
+    W = np.linalg.inv(S)
+    dS = dt.dense_buffer(coeffs, 0.0, 1.0, dy, 0.0)
+    X = 0.7 * (W + W @ dS @ W)
+    assert rel(gX, X) < 1e-9 and rel(gX, oX) < 1e-10
+    AX = np.array([np.sum(c.dense() * X) for c in coeffs[1:]])
+    assert rel(gst[0], AX) < 1e-9
+    assert abs(gst[1] - np.sum(coeffs[0].dense() * X)) <= 1e-9 * max(1.0, np.abs(X).sum())
+    assert abs(gst[3] - np.trace(X)) <= 1e-10 * max(1.0, abs(np.trace(X)))
