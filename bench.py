@@ -1,0 +1,188 @@
+"""Benchmark of the HDSDP dual-scaling interior-point path on one MI355X.
+
+Workload: Lovasz theta SDP of the seeded Erdos-Renyi graph G(n=400, p=0.1),
+seed 2 (n = 400, m = 8084 constraints: tr X = 1 and X_ij = 0 per edge), solved
+by the plain FP64 GPU path to a 1e-6 relative gap.  One step is one complete
+solve (HDSDPOptimize: presolve, Phase A, Phase B, primal check); ``value`` is
+seconds per IPM iteration over the timed steps.
+
+The JSON line also carries:
+  * ``parity``: status, iterations and the error of y*, pObj and dObj against
+    the reference HDSDP's FP64 result on the same instance (stored fixture);
+  * ``roofline``: the Schur pair-gather kernel (one launch per Schur build),
+    HBM-bound: algorithmic bytes = 8 per Schur entry written (lower triangle
+    of M over the sparse constraints) + the n x n FP64 S^-1 read once,
+    divided by its average launch time from HIP events on the library's
+    stream during the timed steps;
+  * ``phases``: device milliseconds per IPM iteration by phase;
+  * ``cpu_baseline``: the reference HDSDP build (oracle/_ref, MKL) solving the
+    same instance on the host cores, when that build is present.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+REF_RUNNER = os.path.join(ROOT, "oracle", "_ref", "hdsdp_ref_run")
+FIXTURE = os.path.join(ROOT, "tests", "golden", "c2_gap1e-6")
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    return ap.parse_args()
+
+
+def parity(res):
+    with open(FIXTURE + ".json") as f:
+        fix = json.load(f)
+    y0 = np.load(FIXTURE + "_y.npy")
+    return {
+        "vs": "reference HDSDP FP64 (CPU) on the same instance",
+        "status": [res.status, fix["status"]],
+        "iters": [res.iters, fix["iters"]],
+        "dobj_rel_err": abs(res.dobj - fix["dobj"]) / max(1.0, abs(fix["dobj"])),
+        "pobj_rel_err": abs(res.pobj - fix["pobj"]) / max(1.0, abs(fix["pobj"])),
+        "y_rel_err": float(np.linalg.norm(res.y - y0) / np.linalg.norm(y0)),
+        "gap": res.rel_gap,
+    }
+
+
+def cpu_baseline(inst, threads):
+    """Reference HDSDP (built from its sources into oracle/_ref) on the host:
+    one full solve of the same instance to the same 1e-6 gap."""
+    if not os.path.exists(REF_RUNNER):
+        return None
+    from hdsdp_amd import instances as I
+
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "inst.bin")
+        I.write_instance(inst, path)
+        env = dict(os.environ, MKL_NUM_THREADS=str(threads), OMP_NUM_THREADS=str(threads))
+        args = [REF_RUNNER, path, os.path.join(td, "y.bin"), "psdp=0", "relopt=1e-6", "absopt=1e30",
+                f"threads={threads}"]
+        r = subprocess.run(args, capture_output=True, text=True, timeout=600, env=env)
+    if "RESULT " not in r.stdout:
+        return {"error": (r.stdout + r.stderr)[-400:]}
+    res = json.loads(r.stdout.split("RESULT ", 1)[1].splitlines()[0])
+    if res.get("status", -1) < 0 or res["iters"] <= 0:
+        return {"error": "reference solve failed"}
+    return {
+        "value": res["time"] / res["iters"],
+        "unit": "s/iter",
+        "cores": threads,
+        "kind": "reference",
+        "sample": f"one full solve of the same instance to 1e-6 gap ({res['iters']} iterations, "
+                  f"{res['time']:.2f} s, MKL {threads} threads)",
+        "solve_s": res["time"],
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the solver's "
+              "Schur/Cholesky path runs on a single MI355X.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from hdsdp_amd import instances as I
+    from hdsdp_amd.solver import load_library, run_solver
+
+    lib = load_library()
+    inst = I.lovasz_theta(400, 0.1, 2)
+    prec = 1 if args.precision == "mixed" else 0
+    opts = dict(relopt=1e-6, absopt=1e30, precision=prec)
+    for _ in range(args.warmup):
+        run_solver(lib, inst, **opts)
+        print(f"[bench] warmup solve done", file=sys.stderr, flush=True)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    results = []
+    for k in range(args.steps):
+        results.append(run_solver(lib, inst, profile=True, **opts))
+        print(f"[bench] step {k} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+
+    iters = sum(r.iters for r in results)
+    s_per_iter = elapsed / iters
+    prof = {k: [sum(r.profile[k][0] for r in results), sum(r.profile[k][1] for r in results)]
+            for k in results[0].profile}
+
+    m, n = inst.m, inst.n
+    # pair kernel: lower-triangle Schur entries between the small constraints
+    n_small = m - 1  # all but tr X = 1 (n diagonal entries) take the pair path
+    pair_bytes = 8.0 * n_small * (n_small + 1) / 2 + 8.0 * n * n
+    pair_ms, pair_cnt = prof["pairs"]
+    pair_avg_s = pair_ms / 1e3 / max(pair_cnt, 1)
+    achieved = pair_bytes / pair_avg_s / 1e9 if pair_cnt else 0.0
+    mf_ms, mf_cnt = prof["mfactor"]
+    mf_flops = 2.0 * m ** 3 / 3.0  # Cholesky (m^3/3) + triangular inverse (m^3/3)
+
+    line = {
+        "metric": "IPM s/iter + solve s to 1e-6 gap, mixed vs FP64 GPU; Schur TFLOP/s vs peak",
+        "value": s_per_iter,
+        "unit": "s/iter",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == 0 else "f64+f32",
+        "data": "synthetic (seeded Erdos-Renyi graph, numpy PCG64 seed 2)",
+        "config": {"workload": "Lovasz theta SDP, G(n=400, p=0.1) seed 2: n=400, m=8084, FP64 GPU, "
+                               "gap 1e-6", "n": n, "m": m, "precision": args.precision},
+        "iters_per_solve": results[0].iters,
+        "solve_s": elapsed / args.steps,
+        "solve_s_to_gap": float(np.mean([r.seconds_to_gap for r in results])),
+        "parity": parity(results[-1]),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "schur_pair_kernel",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_launch": pair_bytes,
+            "avg_launch_us": pair_avg_s * 1e6,
+            "launches": pair_cnt,
+        },
+        "schur_factor": {
+            "bound": "mfma",
+            "achieved_tflops": (mf_flops * mf_cnt / (mf_ms / 1e3) / 1e12) if mf_ms > 0 else 0.0,
+            "peak_tflops": FP64_PEAK_TFLOPS,
+            "avg_ms": mf_ms / max(mf_cnt, 1),
+            "count": mf_cnt,
+        },
+        "phases_ms_per_iter": {k: v[0] / iters for k, v in prof.items()},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(inst, args.cpu_threads)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

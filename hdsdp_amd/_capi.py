@@ -20,16 +20,21 @@ _API = {
     "HDSDPSetDblParamAny": (None, [P_VOID, C.c_int, C.c_double]),
     "HDSDPSetPrecision": (None, [P_VOID, C.c_int]),
     "HDSDPSetVerbose": (None, [P_VOID, C.c_int]),
+    "HDSDPSetProfiling": (None, [P_VOID, C.c_int]),
+    "HDSDPGetProfile": (C.c_int, [P_VOID, P_DBL, P_DBL, C.c_int]),
     "HDSDPGetRowDual": (C.c_int, [P_VOID, P_DBL, P_DBL, P_DBL]),
     "HDSDPGetConeValues": (None, [P_VOID, C.c_int, P_DBL, P_DBL, P_DBL]),
     "HDSDPCheckSolution": (C.c_int, [P_VOID, P_DBL]),
     "HDSDPGetDimacsErrors": (None, [P_VOID, P_DBL]),
     "HDSDPGetIterLog": (C.c_int, [P_VOID, C.c_int, P_DBL]),
     "HDSDPGetSummary": (None, [P_VOID, C.c_double, P_INT, P_INT, P_DBL, P_DBL, P_DBL]),
+    "HDSDPClear": (None, [P_VOID]),
     "HDSDPDestroy": (None, [C.POINTER(P_VOID)]),
     "HUserDataCreate": (C.c_int, [C.POINTER(P_VOID)]),
     "HUserDataSetConeData": (None, [P_VOID, C.c_int, C.c_int, C.c_int, P_INT, P_INT, P_DBL]),
     "HUserDataSetRankOneConeData": (None, [P_VOID, C.c_int, C.c_int, C.c_int, P_INT, P_DBL, P_DBL, P_DBL]),
+    "HUserDataChooseCone": (C.c_int, [P_VOID]),
+    "HUserDataClear": (None, [P_VOID]),
     "HUserDataDestroy": (None, [C.POINTER(P_VOID)]),
     # device ABI
     "hdsdp_dev_create": (C.c_int, [C.POINTER(P_VOID), C.c_int, C.c_int, C.c_int]),
@@ -54,8 +59,13 @@ _API = {
     "hdsdp_dev_get_dual": (C.c_int, [P_VOID, P_DBL]),
     "hdsdp_dev_adjoint": (C.c_int, [P_VOID, C.c_double, P_DBL, P_DBL, P_DBL]),
     "hdsdp_dev_stats": (C.c_int, [P_VOID, P_DBL, C.c_int]),
+    "hdsdp_dev_set_profiling": (C.c_int, [P_VOID, C.c_int]),
+    "hdsdp_dev_get_profile": (C.c_int, [P_VOID, P_DBL, P_DBL, C.c_int]),
     "hdsdp_dev_backend_name": (C.c_char_p, []),
 }
+
+# Device-time profile slots (HDSDP_PROF_* in include/hdsdp_gpu.h)
+PROF_SLOTS = ("schur", "pairs", "mfactor", "cg", "sfactor", "lanczos", "buffer", "primal")
 
 # Entry points every build of the library exports (include/hdsdp.h, include/hdsdp_gpu.h).
 EXPORTED = tuple(_API)

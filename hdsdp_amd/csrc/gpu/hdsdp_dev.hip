@@ -51,6 +51,11 @@ struct hdsdp_dev {
     int useJacobi, paramUseJacobi, maxIter, nRestart;
     double absTol, relTol;
     double stats[6];
+
+    // device-time profile (events on st)
+    bool prof;
+    hipEvent_t evA[HDSDP_PROF_SLOTS], evB[HDSDP_PROF_SLOTS];
+    double profMs[HDSDP_PROF_SLOTS], profCnt[HDSDP_PROF_SLOTS];
 };
 
 namespace {
@@ -117,6 +122,21 @@ hdsdp_retcode read_scalars(hdsdp_dev *d) {
 }
 
 int fidx(int whichBuffer) { return whichBuffer == 0 ? 0 : 1; }
+
+void prof_begin(hdsdp_dev *d, int k) {
+    if (d->prof) (void) hipEventRecord(d->evA[k], d->st);
+}
+
+hdsdp_retcode prof_end(hdsdp_dev *d, int k) {
+    if (!d->prof) return HDSDP_RETCODE_OK;
+    HIP_OK(hipEventRecord(d->evB[k], d->st));
+    HIP_OK(hipEventSynchronize(d->evB[k]));
+    float ms = 0.0f;
+    HIP_OK(hipEventElapsedTime(&ms, d->evA[k], d->evB[k]));
+    d->profMs[k] += ms;
+    d->profCnt[k] += 1.0;
+    return HDSDP_RETCODE_OK;
+}
 
 hdsdp_retcode ensure_linv(hdsdp_dev *d, int f) {
     if (d->liVer[f] == d->facVer[f]) return HDSDP_RETCODE_OK;
@@ -229,6 +249,7 @@ hdsdp_retcode build_precond(hdsdp_dev *d) {
             DCALL(dalloc(d, &d->cf2, (size_t) m));
             DCALL(dalloc(d, &d->workf, (size_t) m * m));
         }
+        prof_begin(d, HDSDP_PROF_MFACTOR);
         hdk::convert_lower_to_float(d->st, m, d->M, m, d->Mf, m);
         hdk::potrf<float>(d->st, m, d->Mf, m, d->workf, d->info_d);
         HIP_OK(hipMemcpyAsync(d->hinfo, d->info_d, sizeof(int), hipMemcpyDeviceToHost, d->st));
@@ -236,13 +257,15 @@ hdsdp_retcode build_precond(hdsdp_dev *d) {
         if (*d->hinfo != 0) return HDSDP_RETCODE_FAILED;
         hdk::trtri<float>(d->st, m, d->Mf, m, d->MLif, m, d->workf);
         hdk::transpose<float>(d->st, m, d->MLif, m, d->MLiTf, m);
-        return check_launch();
+        DCALL(check_launch());
+        return prof_end(d, HDSDP_PROF_MFACTOR);
     }
     if (!d->Mchol) {
         DCALL(dalloc(d, &d->Mchol, (size_t) m * m));
         DCALL(dalloc(d, &d->MLi, (size_t) m * m));
         DCALL(dalloc(d, &d->MLiT, (size_t) m * m));
     }
+    prof_begin(d, HDSDP_PROF_MFACTOR);
     HIP_OK(hipMemcpyAsync(d->Mchol, d->M, sizeof(double) * (size_t) m * m, hipMemcpyDeviceToDevice, d->st));
     hdk::potrf<double>(d->st, m, d->Mchol, m, d->MLiT, d->info_d);
     HIP_OK(hipMemcpyAsync(d->hinfo, d->info_d, sizeof(int), hipMemcpyDeviceToHost, d->st));
@@ -250,7 +273,8 @@ hdsdp_retcode build_precond(hdsdp_dev *d) {
     if (*d->hinfo != 0) return HDSDP_RETCODE_FAILED;
     hdk::trtri<double>(d->st, m, d->Mchol, m, d->MLi, m, d->MLiT);
     hdk::transpose<double>(d->st, m, d->MLi, m, d->MLiT, m);
-    return check_launch();
+    DCALL(check_launch());
+    return prof_end(d, HDSDP_PROF_MFACTOR);
 }
 
 // z = P^-1 r (and optionally p = z)
@@ -402,6 +426,11 @@ hdsdp_retcode hdsdp_dev_create(hdsdp_dev **pdev, int nRow, int nCol, int precisi
     d->nRestart = 20;
     hdk::fill(d->st, d->M, (long) m * m, 0.0);
     for (int k = 0; k < 3; ++k) hdk::fill(d->st, d->S[k], (long) nn, 0.0);
+    d->prof = false;
+    for (int k = 0; k < HDSDP_PROF_SLOTS; ++k) {
+        d->evA[k] = d->evB[k] = nullptr;
+        d->profMs[k] = d->profCnt[k] = 0.0;
+    }
     return check_launch();
 }
 
@@ -609,6 +638,10 @@ void hdsdp_dev_destroy(hdsdp_dev **pdev) {
     for (double *p : hp)
         if (p) hipHostFree(p);
     if (d->hinfo) hipHostFree(d->hinfo);
+    for (int k = 0; k < HDSDP_PROF_SLOTS; ++k) {
+        if (d->evA[k]) hipEventDestroy(d->evA[k]);
+        if (d->evB[k]) hipEventDestroy(d->evB[k]);
+    }
     if (d->st) hipStreamDestroy(d->st);
     delete d;
     *pdev = nullptr;
@@ -617,6 +650,26 @@ void hdsdp_dev_destroy(hdsdp_dev **pdev) {
 hdsdp_retcode hdsdp_dev_stats(hdsdp_dev *d, double *stats, int nStats) {
     for (int k = 0; k < nStats && k < 6; ++k) stats[k] = d->stats[k];
     return HDSDP_RETCODE_OK;
+}
+
+hdsdp_retcode hdsdp_dev_set_profiling(hdsdp_dev *d, int enable) {
+    if (enable && !d->evA[0]) {
+        for (int k = 0; k < HDSDP_PROF_SLOTS; ++k) {
+            HIP_OK(hipEventCreate(&d->evA[k]));
+            HIP_OK(hipEventCreate(&d->evB[k]));
+        }
+    }
+    d->prof = enable != 0;
+    return HDSDP_RETCODE_OK;
+}
+
+int hdsdp_dev_get_profile(hdsdp_dev *d, double *ms, double *counts, int nSlots) {
+    int n = std::min(nSlots, (int) HDSDP_PROF_SLOTS);
+    for (int k = 0; k < n; ++k) {
+        if (ms) ms[k] = d->profMs[k];
+        if (counts) counts[k] = d->profCnt[k];
+    }
+    return n;
 }
 
 // ---- dual buffers -----------------------------------------------------------------
@@ -629,15 +682,19 @@ hdsdp_retcode hdsdp_dev_update_buffer(hdsdp_dev *d, double cCoef, double aScal, 
     for (int i = 0; i < m; ++i) d->hcoef[i + 1] = aScal * aCoef[i];
     HIP_OK(hipMemcpyAsync(d->coef_d, d->hcoef, sizeof(double) * (m + 1), hipMemcpyHostToDevice, d->st));
     d->stagingBusy = true;
+    prof_begin(d, HDSDP_PROF_BUFFER);
     hdk::assemble_buffer(d->st, d->c, d->coef_d, cCoef, d->S[which], dEyeCoef, d->work);
-    return check_launch();
+    DCALL(check_launch());
+    return prof_end(d, HDSDP_PROF_BUFFER);
 }
 
 hdsdp_retcode hdsdp_dev_factorize(hdsdp_dev *d, int which, int *isPsd) {
     const int n = d->n, f = fidx(which);
+    prof_begin(d, HDSDP_PROF_SFACTOR);
     HIP_OK(hipMemcpyAsync(d->L[f], d->S[f], sizeof(double) * (size_t) n * n, hipMemcpyDeviceToDevice, d->st));
     hdk::potrf<double>(d->st, n, d->L[f], n, d->T1, d->info_d);
     HIP_OK(hipMemcpyAsync(d->hinfo, d->info_d, sizeof(int), hipMemcpyDeviceToHost, d->st));
+    DCALL(prof_end(d, HDSDP_PROF_SFACTOR));
     DCALL(sync(d));
     d->facVer[f] += 1;
     d->stats[1] += 1;
@@ -672,9 +729,11 @@ hdsdp_retcode hdsdp_dev_ratio_test(hdsdp_dev *d, int which, double *maxStep) {
         *maxStep = (ds > 0.0) ? HDSDP_INFINITY : (-s / ds);
         return HDSDP_RETCODE_OK;
     }
+    prof_begin(d, HDSDP_PROF_LANCZOS);
     DCALL(ensure_linv(d, f));
     hdk::mirror_lower<double>(d->st, d->n, d->S[2], d->n);
-    return lanczos(d, f, maxStep);
+    DCALL(lanczos(d, f, maxStep));
+    return prof_end(d, HDSDP_PROF_LANCZOS);
 }
 
 // ---- Schur complement ---------------------------------------------------------------
@@ -683,6 +742,7 @@ hdsdp_retcode hdsdp_dev_schur(hdsdp_dev *d, int typeKKT, double Rd, double *asin
                               double *traceSinv) {
     const int m = d->m, n = d->n;
     if (typeKKT == 2 || typeKKT == 3) return HDSDP_RETCODE_FAILED;
+    if (typeKKT != 1) prof_begin(d, HDSDP_PROF_SCHUR);
     DCALL(ensure_linv(d, 0));
     // W = S^-1 = L^-T L^-1
     hdk::gemm<double>(d->st, true, false, false, n, n, n, 1.0, d->Li[0], n, 0, d->Li[0], n, 0, 0.0, d->W, n, 0, 1);
@@ -693,7 +753,9 @@ hdsdp_retcode hdsdp_dev_schur(hdsdp_dev *d, int typeKKT, double Rd, double *asin
     if (typeKKT != 1 && Rd != 0.0) hdk::trace_diag(d->st, n, d->W, d->sc_d + hdk::SC_AUX);
     if (typeKKT != 1) {
         if (d->hasZero) hdk::fill(d->st, d->M, (long) m * m, 0.0);
+        prof_begin(d, HDSDP_PROF_PAIRS);
         hdk::schur_pairs(d->st, d->c, d->W, d->M, m);
+        DCALL(prof_end(d, HDSDP_PROF_PAIRS));
         for (int h : d->heavy) {
             // B_h = W A_h W, then M(h, k) = <A_k, B_h>
             hdk::densify(d->st, d->c, h, d->T1);
@@ -703,6 +765,7 @@ hdsdp_retcode hdsdp_dev_schur(hdsdp_dev *d, int typeKKT, double Rd, double *asin
             hdk::scatter_row(d->st, m, h - 1, d->ctmp, d->M, m);
         }
         d->stats[0] += 1;
+        DCALL(prof_end(d, HDSDP_PROF_SCHUR));
     }
     DCALL(check_launch());
     DCALL(download(d, asinv, d->vecA_d, m, d->hvecA));
@@ -746,7 +809,9 @@ hdsdp_retcode hdsdp_dev_kkt_factorize(hdsdp_dev *d) {
 
 hdsdp_retcode hdsdp_dev_kkt_solve(hdsdp_dev *d, const double *rhs, double *sol) {
     DCALL(upload(d, d->crhs, rhs, d->m, d->hvecB));
+    prof_begin(d, HDSDP_PROF_CG);
     int rc = cg_solve(d);
+    DCALL(prof_end(d, HDSDP_PROF_CG));
     if (rc < 0) return HDSDP_RETCODE_FAILED;
     if (rc == 1) return HDSDP_RETCODE_OK;  // tiny right-hand side: solution left as it was
     return download(d, sol, d->cx, d->m, d->hvecB);
@@ -765,6 +830,7 @@ hdsdp_retcode hdsdp_dev_get_schur(hdsdp_dev *d, double *M) {
 hdsdp_retcode hdsdp_dev_primal(hdsdp_dev *d, double mu, const double *y, const double *dy, int *ok) {
     (void) y;
     const int n = d->n;
+    prof_begin(d, HDSDP_PROF_PRIMAL);
     DCALL(hdsdp_dev_factorize(d, 1, ok));
     if (!*ok) return HDSDP_RETCODE_OK;
     DCALL(hdsdp_dev_update_buffer(d, 0.0, 1.0, dy, 0.0, 2));
@@ -777,7 +843,8 @@ hdsdp_retcode hdsdp_dev_primal(hdsdp_dev *d, double mu, const double *y, const d
     hdk::gemm<double>(d->st, true, false, false, n, n, n, 1.0, d->Li[1], n, 0, d->T1, n, 0, 0.0, d->T2, n, 0, 1);
     hdk::gemm<double>(d->st, false, false, false, n, n, n, 1.0, d->T2, n, 0, d->Li[1], n, 0, 0.0, d->T1, n, 0, 1);
     hdk::sym_avg(d->st, n, d->T1, mu, 0.0, d->X);
-    return check_launch();
+    DCALL(check_launch());
+    return prof_end(d, HDSDP_PROF_PRIMAL);
 }
 
 hdsdp_retcode hdsdp_dev_primal_stats(hdsdp_dev *d, double *AX, double *traceCX, double *XdotS, double *traceX) {
@@ -795,6 +862,7 @@ hdsdp_retcode hdsdp_dev_primal_stats(hdsdp_dev *d, double *AX, double *traceCX, 
 }
 
 hdsdp_retcode hdsdp_dev_get_primal(hdsdp_dev *d, double *X) {
+    DCALL(sync(d));
     HIP_OK(hipMemcpy(X, d->X, sizeof(double) * (size_t) d->n * d->n, hipMemcpyDeviceToHost));
     return HDSDP_RETCODE_OK;
 }

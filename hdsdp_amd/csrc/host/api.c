@@ -349,6 +349,15 @@ void HDSDPSetVerbose( hdsdp *s, int verbose ) {
     s->verbose = verbose;
 }
 
+void HDSDPSetProfiling( hdsdp *s, int enable ) {
+    s->profiling = enable ? 1 : 0;
+}
+
+int HDSDPGetProfile( hdsdp *s, double *ms, double *counts, int nSlots ) {
+    if ( !s->sdp || !s->sdp->dev ) return 0;
+    return hdsdp_dev_get_profile(s->sdp->dev, ms, counts, nSlots);
+}
+
 /* interface/hdsdp.c:617-689 */
 hdsdp_retcode HDSDPOptimize( hdsdp *s, int dOptOnly ) {
     hdsdp_retcode retcode = HDSDP_RETCODE_OK;
@@ -363,6 +372,7 @@ hdsdp_retcode HDSDPOptimize( hdsdp *s, int dOptOnly ) {
     }
     hlog(s, "Pre-solver starts \n");
     HCALL(sdp_cone_create(&s->sdp, s->userCones[0], s->precision));
+    if ( s->profiling ) HCALL(hdsdp_dev_set_profiling(s->sdp->dev, 1));
     HCALL(sdp_cone_presolve(s->sdp));
     HCALL(kkt_init(s->kkt, s->sdp));
     get_statistics(s);
@@ -501,9 +511,10 @@ void HDSDPClear( hdsdp *s ) {
     free(s->dAccRowDualMaker); free(s->dAccRowDualStepMaker);
     free(s->dInaccRowDualMaker); free(s->dInaccRowDualStepMaker);
     free(s->iterLog);
-    int verbose = s->verbose;
+    int verbose = s->verbose, profiling = s->profiling;
     memset(s, 0, sizeof(hdsdp));
     s->verbose = verbose;
+    s->profiling = profiling;
 }
 
 void HDSDPDestroy( hdsdp **pHSolver ) {

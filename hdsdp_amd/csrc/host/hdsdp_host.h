@@ -155,6 +155,7 @@ struct hdsdp_solver_internal {
     host_kkt *kkt;
     int precision;
     int verbose;
+    int profiling;
 
     double *dRowDual;
     double *dRowDualStep;

@@ -20,7 +20,9 @@ __all__ = [
     "SDPInstance",
     "packed_index",
     "maxcut",
+    "maxcut_graph",
     "lovasz_theta",
+    "theta_graph",
     "dense_random",
     "rank_one",
     "write_instance",
@@ -103,6 +105,27 @@ def _random_graph(rng, n, p):
     return iu[keep], ju[keep]
 
 
+def maxcut_graph(n: int, i, j, w, name: str = "maxcut") -> SDPInstance:
+    """MaxCut relaxation of the weighted graph with edges ``(i[k], j[k])``, ``i < j``:
+    ``C = -L/4`` (``L`` the weighted Laplacian), ``diag(X) = 1``."""
+    i = np.asarray(i, dtype=np.int64)
+    j = np.asarray(j, dtype=np.int64)
+    w = np.asarray(w, dtype=np.float64)
+    deg = np.zeros(n)
+    np.add.at(deg, i, w)
+    np.add.at(deg, j, w)
+    # C = -(D - W)/4: diagonal -deg/4, off-diagonal +w/4 at (max, min)
+    rows = np.concatenate([np.arange(n), j])  # j > i so (j, i) is lower
+    colsj = np.concatenate([np.arange(n), i])
+    vals = np.concatenate([-deg / 4.0, w / 4.0])
+    nz = vals != 0.0
+    cpk = packed_index(n, rows[nz], colsj[nz])
+    cols = [(cpk, vals[nz])]
+    for k in range(n):
+        cols.append((np.array([packed_index(n, k, k)]), np.array([1.0])))
+    return _assemble(name, n, cols, np.ones(n))
+
+
 def maxcut(n: int, p: float, seed: int, weights: str = "pm1") -> SDPInstance:
     """MaxCut relaxation on G(n, p): ``C = -L/4``, ``diag(X) = 1``.
 
@@ -116,25 +139,12 @@ def maxcut(n: int, p: float, seed: int, weights: str = "pm1") -> SDPInstance:
         w = rng.random(i.shape[0])
     else:
         raise ValueError(weights)
-    deg = np.zeros(n)
-    np.add.at(deg, i, w)
-    np.add.at(deg, j, w)
-    # C = -(D - W)/4: diagonal -deg/4, off-diagonal +w/4 at (max, min)
-    rows = np.concatenate([np.arange(n), j])  # j > i so (j, i) is lower
-    colsj = np.concatenate([np.arange(n), i])
-    vals = np.concatenate([-deg / 4.0, w / 4.0])
-    nz = vals != 0.0
-    cpk = packed_index(n, rows[nz], colsj[nz])
-    cols = [(cpk, vals[nz])]
-    for k in range(n):
-        cols.append((np.array([packed_index(n, k, k)]), np.array([1.0])))
-    return _assemble(f"maxcut_n{n}_p{p}_s{seed}", n, cols, np.ones(n))
+    return maxcut_graph(n, i, j, w, f"maxcut_n{n}_p{p}_s{seed}")
 
 
-def lovasz_theta(n: int, p: float, seed: int) -> SDPInstance:
-    """Lovasz theta SDP on G(n, p): ``C = -J``, ``tr X = 1``, ``X_ij = 0`` per edge."""
-    rng = np.random.default_rng(seed)
-    i, j = _random_graph(rng, n, p)
+def theta_graph(n: int, i, j, name: str = "theta") -> SDPInstance:
+    """Lovasz theta SDP of the graph with edges ``(i[k], j[k])``, ``i < j``:
+    ``C = -J``, ``tr X = 1``, ``X_ij = 0`` per edge; the optimum is ``-theta``."""
     npk = n * (n + 1) // 2
     cols = [(np.arange(npk), -np.ones(npk))]
     cols.append((packed_index(n, np.arange(n), np.arange(n)), np.ones(n)))
@@ -142,7 +152,14 @@ def lovasz_theta(n: int, p: float, seed: int) -> SDPInstance:
         cols.append((np.array([packed_index(n, c, a)]), np.array([1.0])))
     b = np.zeros(len(cols) - 1)
     b[0] = 1.0
-    return _assemble(f"theta_n{n}_p{p}_s{seed}", n, cols, b)
+    return _assemble(name, n, cols, b)
+
+
+def lovasz_theta(n: int, p: float, seed: int) -> SDPInstance:
+    """Lovasz theta SDP on G(n, p): ``C = -J``, ``tr X = 1``, ``X_ij = 0`` per edge."""
+    rng = np.random.default_rng(seed)
+    i, j = _random_graph(rng, n, p)
+    return theta_graph(n, i, j, f"theta_n{n}_p{p}_s{seed}")
 
 
 def dense_random(n: int, m: int, seed: int) -> SDPInstance:

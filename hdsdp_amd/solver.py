@@ -60,6 +60,7 @@ class SolveResult:
     seconds_to_gap: float
     dimacs: np.ndarray
     log: np.ndarray  # records x 9: phase, iter, pObj, dObj, inf, mu, step, prox, t
+    profile: dict | None = None  # slot -> (device ms, timed regions) when profiling
 
     @property
     def status_name(self) -> str:
@@ -73,7 +74,7 @@ class SolveResult:
 def run_solver(lib: C.CDLL, inst: SDPInstance, *, psdp: int = 0, relopt: float | None = None,
                absopt: float | None = None, maxiter: int | None = None,
                timelimit: float | None = None, precision: int = 0, verbose: bool = False,
-               gap_target: float = 1e-6, keep=None) -> SolveResult:
+               gap_target: float = 1e-6, profile: bool = False, keep=None) -> SolveResult:
     """Solve ``inst`` through the C API of ``lib``.  ``keep`` (a list) receives the
     solver handle instead of destroying it, for callers that query it further."""
     m, n = inst.m, inst.n
@@ -116,6 +117,7 @@ def run_solver(lib: C.CDLL, inst: SDPInstance, *, psdp: int = 0, relopt: float |
         if timelimit is not None:
             lib.HDSDPSetDblParam(solver, DBL_PARAM["timelimit"], float(timelimit))
         lib.HDSDPSetPrecision(solver, int(precision))
+        lib.HDSDPSetProfiling(solver, 1 if profile else 0)
         rc = lib.HDSDPOptimize(solver, 1)
         y = np.zeros(m)
         pobj, dobj = C.c_double(), C.c_double()
@@ -130,8 +132,14 @@ def run_solver(lib: C.CDLL, inst: SDPInstance, *, psdp: int = 0, relopt: float |
             lib.HDSDPGetIterLog(solver, nlog, _capi.dptr(log))
         dimacs = np.zeros(6)
         lib.HDSDPGetDimacsErrors(solver, _capi.dptr(dimacs))
+        prof = None
+        if profile:
+            ns = len(_capi.PROF_SLOTS)
+            ms, cnt = np.zeros(ns), np.zeros(ns)
+            lib.HDSDPGetProfile(solver, _capi.dptr(ms), _capi.dptr(cnt), ns)
+            prof = {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(_capi.PROF_SLOTS)}
         res = SolveResult(int(rc), int(st.value), int(it.value), y, pobj.value, dobj.value, mu.value,
-                          secs.value, secs_gap.value, dimacs, log)
+                          secs.value, secs_gap.value, dimacs, log, prof)
         if keep is not None:
             keep.append((solver, ud, arrays))
             solver = None

@@ -140,6 +140,12 @@ void HDSDPGetDimacsErrors( hdsdp *HSolver, double errs[6] );
 /* Silence the iteration log (the reference prints unconditionally). */
 void HDSDPSetVerbose( hdsdp *HSolver, int verbose );
 
+/* Device-time profile of the last HDSDPOptimize (slots HDSDP_PROF_* of
+ * hdsdp_gpu.h); HDSDPSetProfiling(1) before HDSDPOptimize turns it on.
+ * HDSDPGetProfile returns the number of slots filled. */
+void HDSDPSetProfiling( hdsdp *HSolver, int enable );
+int HDSDPGetProfile( hdsdp *HSolver, double *ms, double *counts, int nSlots );
+
 #ifdef __cplusplus
 }
 #endif

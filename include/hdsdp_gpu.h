@@ -156,6 +156,31 @@ hdsdp_retcode hdsdp_dev_adjoint( hdsdp_dev *dev, double mu, const double *dLdy,
  * iterations, [3] Lanczos matvecs, [4] Cholesky of M, [5] last CG iterations. */
 hdsdp_retcode hdsdp_dev_stats( hdsdp_dev *dev, double *stats, int nStats );
 
+/* Device-time profile, measured with events on the library's own stream.
+ * Slot k accumulates milliseconds (ms[k]) and the number of timed regions
+ * (counts[k]) of:
+ *   HDSDP_PROF_SCHUR     whole Schur build (S^-1, traces, M)
+ *   HDSDP_PROF_PAIRS     the pair-gather Schur kernel alone (one launch per build)
+ *   HDSDP_PROF_MFACTOR   Cholesky of M and its triangular inverse
+ *   HDSDP_PROF_CG        one preconditioned CG solve
+ *   HDSDP_PROF_SFACTOR   Cholesky of a dual buffer
+ *   HDSDP_PROF_LANCZOS   one Lanczos step-length call
+ *   HDSDP_PROF_BUFFER    one dual-buffer assembly
+ *   HDSDP_PROF_PRIMAL    primal recovery
+ * Profiling is off by default: enabling it adds one event synchronisation per
+ * timed region.  No counterpart in the reference (measurement only). */
+#define HDSDP_PROF_SCHUR   0
+#define HDSDP_PROF_PAIRS   1
+#define HDSDP_PROF_MFACTOR 2
+#define HDSDP_PROF_CG      3
+#define HDSDP_PROF_SFACTOR 4
+#define HDSDP_PROF_LANCZOS 5
+#define HDSDP_PROF_BUFFER  6
+#define HDSDP_PROF_PRIMAL  7
+#define HDSDP_PROF_SLOTS   8
+hdsdp_retcode hdsdp_dev_set_profiling( hdsdp_dev *dev, int enable );
+int hdsdp_dev_get_profile( hdsdp_dev *dev, double *ms, double *counts, int nSlots );
+
 /* Name of the implementation ("hip-gfx950" for the product library). */
 const char *hdsdp_dev_backend_name( void );
 

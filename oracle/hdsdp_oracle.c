@@ -490,6 +490,9 @@ hdsdp_retcode hdsdp_dev_ratio_test( hdsdp_dev *d, int which, double *maxStep ) {
     lz_normalize(n, lz->v);
     memcpy(lz->V, lz->v, sizeof(double) * n);
     int check = OMIN(lz->maxDim / 5, 3);
+    static int trace = -1;
+    if ( trace < 0 ) trace = getenv("HDSDP_ORACLE_LZ_TRACE") != NULL;
+    if ( trace ) printf("Lanczos call %d\n", lz->nComputed);
     for ( int k = 0; k < lz->maxDim; ++k ) {
         lz_mvec(d, L, lz->v, lz->w);
         if ( k > 0 ) {
@@ -499,6 +502,7 @@ hdsdp_retcode hdsdp_dev_ratio_test( hdsdp_dev *d, int which, double *maxStep ) {
         double alp = -odot(n, lz->w, &VV(0, k));
         for ( int i = 0; i < n; ++i ) lz->w[i] += alp * VV(i, k);
         HH(k, k) = -alp;
+        if ( trace ) printf("Lanczos Alp value: %f \n", -alp);
         memcpy(lz->v, lz->w, sizeof(double) * n);
         double normPres = lz_normalize(n, lz->v);
         if ( normPres > 0.0 ) {
@@ -510,6 +514,7 @@ hdsdp_retcode hdsdp_dev_ratio_test( hdsdp_dev *d, int which, double *maxStep ) {
             for ( int i = 0; i < kp1; ++i ) memcpy(lz->U + (size_t) kp1 * i, &HH(0, i), sizeof(double) * kp1);
             if ( kp1 < 2 || hdsdp_eig_small_top(kp1, lz->U, 2, lz->d, lz->Y) != 0 ) return HDSDP_RETCODE_FAILED;
             double resi = fabs(HH(kp1, k) * lz->Y[kp1 + k]);
+            if ( trace ) printf("Lanczos outer resi value: %f \n", resi);
             if ( resi < 1e-04 || k >= lz->maxDim - 1 ) {
                 double e1 = lz->d[1], e2 = lz->d[0];
                 for ( int i = 0; i < n; ++i ) {
@@ -533,6 +538,7 @@ hdsdp_retcode hdsdp_dev_ratio_test( hdsdp_dev *d, int which, double *maxStep ) {
                 double gamma = ( diff > 0 ) ? diff : 1e-16;
                 double r1sq = r1 * r1 / gamma;
                 gamma = OMIN(r1, r1sq);
+                if ( trace ) printf("Lanczos bound k=%d e1=%.12e e2=%.12e r1=%.6e r2=%.6e gamma=%.6e\n", k, e1, e2, r1, r2, gamma);
                 if ( gamma < 1e-03 || gamma + e1 <= 0.5 ) {
                     *maxStep = ( gamma + e1 <= 0.0 ) ? HDSDP_INFINITY : 1.0 / (gamma + e1);
                     break;
@@ -826,4 +832,20 @@ hdsdp_retcode hdsdp_dev_adjoint( hdsdp_dev *d, double mu, const double *dLdy, do
     for ( size_t k = 0; k < (size_t) n * n; ++k ) dLdC[k] = -dLdC[k];
     free(lam);
     return HDSDP_RETCODE_OK;
+}
+
+/* Profiling slots exist for ABI completeness; the oracle reports no device time. */
+hdsdp_retcode hdsdp_dev_set_profiling( hdsdp_dev *d, int enable ) {
+    (void) d;
+    (void) enable;
+    return HDSDP_RETCODE_OK;
+}
+
+int hdsdp_dev_get_profile( hdsdp_dev *d, double *ms, double *counts, int nSlots ) {
+    (void) d;
+    for ( int k = 0; k < nSlots && k < HDSDP_PROF_SLOTS; ++k ) {
+        if ( ms ) ms[k] = 0.0;
+        if ( counts ) counts[k] = 0.0;
+    }
+    return nSlots < HDSDP_PROF_SLOTS ? nSlots : HDSDP_PROF_SLOTS;
 }

@@ -45,7 +45,7 @@ int main(int argc, char **argv) {
         fprintf(stderr, "usage: %s INSTANCE YOUT [key=value ...]\n", argv[0]);
         return 2;
     }
-    int psdp = 1, maxiter = -1, corra = -1, corrb = -1;
+    int psdp = 1, maxiter = -1, corra = -1, corrb = -1, threads = -1;
     double relopt = -1.0, absopt = -1.0, timelimit = -1.0;
     for (int a = 3; a < argc; ++a) {
         if (!strncmp(argv[a], "psdp=", 5)) psdp = atoi(argv[a] + 5);
@@ -55,6 +55,7 @@ int main(int argc, char **argv) {
         else if (!strncmp(argv[a], "timelimit=", 10)) timelimit = atof(argv[a] + 10);
         else if (!strncmp(argv[a], "corra=", 6)) corra = atoi(argv[a] + 6);
         else if (!strncmp(argv[a], "corrb=", 6)) corrb = atoi(argv[a] + 6);
+        else if (!strncmp(argv[a], "threads=", 8)) threads = atoi(argv[a] + 8);
         else { fprintf(stderr, "unknown option %s\n", argv[a]); return 2; }
     }
 
@@ -89,6 +90,7 @@ int main(int argc, char **argv) {
     if (absopt > 0) HDSDPSetDblParam(solver, DBL_PARAM_ABSOPTTOL, absopt);
     if (maxiter > 0) HDSDPSetIntParam(solver, INT_PARAM_MAXITER, maxiter);
     if (timelimit > 0) HDSDPSetDblParam(solver, DBL_PARAM_TIMELIMIT, timelimit);
+    if (threads > 0) HDSDPSetIntParam(solver, INT_PARAM_THREADS, threads);
     (void) corra; (void) corrb;
     rc = HDSDPOptimize(solver, 1);
     double t1 = now_s();

@@ -1,0 +1,242 @@
+"""Pins the CPU oracle (oracle/hdsdp_oracle.c, linked with the product's host C
+into oracle/_build/libhdsdp_oracle.so) before it is trusted as the checker of
+the HIP library:
+
+* device-level functions against numpy closed forms on random coefficient
+  sets of every kind (buffer assembly, Cholesky/PSD test, log-det, Schur matrix
+  M_ij = <A_i, S^-1 A_j S^-1> and its right-hand sides, CG, Lanczos step
+  bound, primal recovery);
+* whole solves against SDPs with known optimal values (Lovasz theta of C5,
+  the Petersen graph, K_n and the empty graph; MaxCut bounds of C5 and K6);
+* whole solves against the golden traces of the reference build (status,
+  iteration count, log structure, objectives and y*).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import devtools as dt
+from hdsdp_amd import instances as I
+from hdsdp_amd.solver import run_solver
+
+CASES = {
+    "spr1_unit": (37, 45, [dt.SPR1]),
+    "sparse_small": (41, 70, [dt.SPARSE]),
+    "mixed_sparse": (66, 90, [dt.SPARSE, dt.SPR1, "SPARSE_BIG", "SPR1_BIG"]),
+    "dense": (30, 40, [dt.DENSE]),
+    "dsr1": (35, 50, [dt.DSR1]),
+    "all_kinds": (50, 60, [dt.SPARSE, dt.SPR1, dt.DENSE, dt.DSR1, "SPARSE_BIG", dt.ZERO]),
+}
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def make(oracle, name, seed):
+    n, m, kinds = CASES[name]
+    rng = np.random.default_rng(seed)
+    coeffs = dt.rand_coeffs(rng, n, m, kinds)
+    y, eye = dt.interior_point(coeffs, rng)
+    return coeffs, dt.Dev(oracle, coeffs), rng, y, eye
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_buffer_factor_logdet(oracle, name):
+    coeffs, d, rng, y, eye = make(oracle, name, 0)
+    d.update(1.0, -1.0, y, eye, 0)
+    assert d.factorize(0) == 1
+    S = dt.dense_buffer(coeffs, 1.0, -1.0, y, eye)
+    assert rel(d.get_dual(), S) < 1e-13
+    ld = np.linalg.slogdet(S)[1]
+    assert abs(d.logdet(0) - ld) <= 1e-11 * max(1.0, abs(ld))
+    lam = np.linalg.eigvalsh(S).min()
+    d.update(1.0, -1.0, y, eye - lam - 1.0, 1)
+    assert d.factorize(1) == 0
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_schur_closed_form(oracle, name):
+    coeffs, d, rng, y, eye = make(oracle, name, 1)
+    d.update(1.0, -1.0, y, eye, 0)
+    assert d.factorize(0) == 1
+    Rd = -0.37
+    a, b, t = d.schur(0, Rd)
+    M = d.get_schur()
+    W = np.linalg.inv(dt.dense_buffer(coeffs, 1.0, -1.0, y, eye))
+    A = [c.dense() for c in coeffs[1:]]
+    Mref = np.array([[np.sum(Ai * (W @ Aj @ W)) for Aj in A] for Ai in A])
+    assert rel(M, Mref) < 1e-10
+    assert rel(a, [np.sum(Ai * W) for Ai in A]) < 1e-10
+    assert rel(b, [Rd * np.sum(Ai * (W @ W)) for Ai in A]) < 1e-10
+    assert abs(t - np.trace(W)) <= 1e-12 * abs(np.trace(W))
+
+
+def test_schur_catches_transposed_sandwich(oracle):
+    """A plausible slip (W A_j instead of W A_j W) changes M far beyond the tolerance."""
+    coeffs, d, rng, y, eye = make(oracle, "all_kinds", 1)
+    d.update(1.0, -1.0, y, eye, 0)
+    d.factorize(0)
+    d.schur(0, 0.0)
+    M = d.get_schur()
+    W = np.linalg.inv(dt.dense_buffer(coeffs, 1.0, -1.0, y, eye))
+    A = [c.dense() for c in coeffs[1:]]
+    wrong = np.array([[np.sum(Ai * (W @ Aj)) for Aj in A] for Ai in A])
+    assert rel(M, wrong) > 1e-3
+
+
+@pytest.mark.parametrize("name", ["spr1_unit", "mixed_sparse", "all_kinds"])
+def test_cg_solve(oracle, name):
+    coeffs, d, rng, y, eye = make(oracle, name, 2)
+    m = len(coeffs) - 1
+    diag = rng.uniform(0.1, 1.0, m)
+    rhs = rng.standard_normal(m)
+    d.update(1.0, -1.0, y, eye, 0)
+    d.factorize(0)
+    d.schur(0, 0.0)
+    d.add_diag(diag)
+    d.regularize(1e-6)
+    d.kkt_params(5e-12, 1e-12)
+    assert d.kkt_factorize() == 0
+    x = d.kkt_solve(rhs)
+    assert rel(x, np.linalg.solve(d.get_schur(), rhs)) < 1e-7
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_lanczos_bound(oracle, name):
+    coeffs, d, rng, y, eye = make(oracle, name, 3)
+    dy = rng.standard_normal(len(coeffs) - 1)
+    d.update(1.0, -1.0, y, eye, 0)
+    d.factorize(0)
+    d.update(0.0, -1.0, dy, 0.3, 2)
+    step = d.ratio(0)
+    S = dt.dense_buffer(coeffs, 1.0, -1.0, y, eye)
+    dS = dt.dense_buffer(coeffs, 0.0, -1.0, dy, 0.3)
+    Li = np.linalg.inv(np.linalg.cholesky(S))
+    lam = np.linalg.eigvalsh(Li @ (-dS) @ Li.T).max()
+    assert lam > 0
+    # safeguarded: never beyond the boundary, and close to it
+    assert step <= (1.0 / lam) * (1 + 1e-10)
+    assert step >= 0.9 / lam
+
+
+@pytest.mark.parametrize("name", ["spr1_unit", "all_kinds"])
+def test_primal_recovery(oracle, name):
+    coeffs, d, rng, y, eye = make(oracle, name, 4)
+    m = len(coeffs) - 1
+    dy = 0.01 * rng.standard_normal(m)
+    d.update(1.0, -1.0, y, eye, 1)
+    assert d.primal(0.7, y, dy) == 1
+    X = d.get_primal()
+    W = np.linalg.inv(dt.dense_buffer(coeffs, 1.0, -1.0, y, eye))
+    dS = dt.dense_buffer(coeffs, 0.0, 1.0, dy, 0.0)
+    Xref = 0.7 * (W + W @ dS @ W)
+    assert rel(X, Xref) < 1e-10
+    AX, cx, xs, tx = d.primal_stats()
+    assert rel(AX, [np.sum(c.dense() * Xref) for c in coeffs[1:]]) < 1e-10
+    assert abs(tx - np.trace(Xref)) <= 1e-10 * abs(np.trace(Xref))
+
+
+def test_eig_small_matches_numpy(oracle):
+    import ctypes as C
+
+    fn = oracle.hdsdp_eig_small_top
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    rng = np.random.default_rng(5)
+    for n in (2, 3, 7, 19, 31):
+        A = rng.standard_normal((n, n))
+        A = A + A.T
+        ev = np.zeros(2)
+        V = np.zeros(2 * n)
+        assert fn(n, np.asfortranarray(A).ctypes.data_as(C.POINTER(C.c_double)), 2,
+                  ev.ctypes.data_as(C.POINTER(C.c_double)), V.ctypes.data_as(C.POINTER(C.c_double))) == 0
+        w, U = np.linalg.eigh(A)
+        assert np.allclose(ev, w[-2:], rtol=1e-12, atol=1e-12)
+        for k in range(2):
+            v = V[k * n:(k + 1) * n]
+            assert abs(abs(v @ U[:, n - 2 + k]) - 1.0) < 1e-10
+
+
+# ---- known optimal values ---------------------------------------------------------------
+
+def _cycle(n):
+    i = np.arange(n)
+    j = (i + 1) % n
+    return np.minimum(i, j), np.maximum(i, j)
+
+
+def _petersen():
+    outer = [(k, (k + 1) % 5) for k in range(5)]
+    spokes = [(k, k + 5) for k in range(5)]
+    inner = [(5 + k, 5 + (k + 2) % 5) for k in range(5)]
+    e = np.array(outer + spokes + inner)
+    return np.minimum(e[:, 0], e[:, 1]), np.maximum(e[:, 0], e[:, 1])
+
+
+def _complete(n):
+    return np.triu_indices(n, 1)
+
+
+KNOWN = {
+    # Lovasz theta: minimum of <-J, X> is -theta(G)
+    "theta_C5": (lambda: I.theta_graph(5, *_cycle(5)), -np.sqrt(5.0)),
+    "theta_petersen": (lambda: I.theta_graph(10, *_petersen()), -4.0),
+    "theta_K7": (lambda: I.theta_graph(7, *_complete(7)), -1.0),
+    "theta_C7": (lambda: I.theta_graph(7, *_cycle(7)), -7 * np.cos(np.pi / 7) / (1 + np.cos(np.pi / 7))),
+    # MaxCut relaxation value n/2 (1 - cos(pi (n-1)/n)) on odd cycles, n^2/4 on K_n
+    "maxcut_C5": (lambda: I.maxcut_graph(5, *_cycle(5), np.ones(5)), -(25 + 5 * np.sqrt(5.0)) / 8),
+    "maxcut_K6": (lambda: I.maxcut_graph(6, *_complete(6), np.ones(15)), -9.0),
+}
+
+
+@pytest.mark.parametrize("name", list(KNOWN))
+def test_known_optimum(oracle, name):
+    make_inst, opt = KNOWN[name]
+    res = run_solver(oracle, make_inst(), relopt=1e-9, absopt=1e-9)
+    assert res.status == 3
+    assert abs(res.dobj - opt) <= 1e-6 * max(1.0, abs(opt))
+    assert abs(res.pobj - opt) <= 1e-6 * max(1.0, abs(opt))
+
+
+# ---- reference golden traces ----------------------------------------------------------
+
+GOLDEN_CASES = {
+    # name: (pobj tol, y tol)
+    "theta_n50": (1e-8, 1e-6),
+    "maxcut_n60": (1e-8, 1e-6),
+    "dense_n20_m30": (1e-7, 1e-6),
+    "dense_n40_m60": (1e-8, 1e-6),
+    "c1_default": (1e-8, 1e-7),
+    "c1_gap1e-6": (1e-7, 1e-5),
+}
+
+
+@pytest.mark.parametrize("name", list(GOLDEN_CASES))
+def test_matches_reference_golden(oracle, name):
+    ptol, ytol = GOLDEN_CASES[name]
+    inst, fix, res = dt.solve_like_golden(oracle, name)
+    rep = dt.parity_report(fix, res)
+    assert res.status == fix["status"] and res.iters == fix["iters"], rep
+    assert rep["dobj_rel"] <= 1e-8 and rep["pobj_rel"] <= ptol and rep["y_rel"] <= ytol, rep
+    assert [(int(r[0]), int(r[1])) for r in res.log] == [(int(r[0]), int(r[1])) for r in fix["log"]]
+
+
+def test_golden_fixtures_are_consistent():
+    """Every fixture's instance still hashes to the recorded digest, and its
+    log ends at the reported objectives."""
+    import golden.make_golden as mg
+
+    for name in mg.CASES:
+        path = os.path.join(dt.GOLDEN, name + ".json")
+        if not os.path.exists(path):
+            continue
+        with open(path) as f:
+            fix = json.load(f)
+        make_inst, _ = mg.CASES[name]
+        assert mg.instance_digest(make_inst()) == fix["digest"], name
+        last = fix["log"][-1]
+        assert abs(last[3] - fix["dobj"]) <= 1e-7 * max(1.0, abs(fix["dobj"])), name
