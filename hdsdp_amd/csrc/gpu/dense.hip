@@ -4,9 +4,12 @@
 // GEMM tiles are 64 x 64 per workgroup of four 64-wide waves; each wave owns a
 // 32 x 32 quadrant as 2 x 2 MFMA 16x16 accumulators and the k-dimension is
 // staged through LDS in slabs of 16.  FP64 uses v_mfma_f64_16x16x4_f64, FP32
-// v_mfma_f32_16x16x4_f32; both take one A and one B element per lane:
+// v_mfma_f32_16x16x4_f32; both take one A and one B element per lane,
 //   A[i][k] at lane i + 16 k,  B[k][j] at lane j + 16 k,
-//   D[i][j] at lane j + 16 (i / 4), accumulator element i % 4.
+// and return four accumulator elements per lane in different row orders
+// (measured on gfx950):
+//   FP32: D[i][j] at lane j + 16 (i / 4), element i % 4;
+//   FP64: D[i][j] at lane j + 16 (i % 4), element i / 4.
 #include "dev_common.h"
 
 namespace hdk {
@@ -17,6 +20,8 @@ struct Mfma;
 template <>
 struct Mfma<double> {
     typedef v4d acc_t;
+    // row of accumulator element r held by lane group g = lane / 16
+    __device__ static inline int row(int g, int r) { return 4 * r + g; }
     __device__ static inline acc_t op(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
@@ -25,6 +30,7 @@ struct Mfma<double> {
 template <>
 struct Mfma<float> {
     typedef v4f acc_t;
+    __device__ static inline int row(int g, int r) { return 4 * g + r; }
     __device__ static inline acc_t op(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
@@ -86,7 +92,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha,
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                int i = i0 + wi + a * 16 + 4 * lk + r;
+                int i = i0 + wi + a * 16 + Mfma<T>::row(lk, r);
                 int j = j0 + wj + b * 16 + lr;
                 if (i < M && j < N && (!LOWER || i >= j)) {
                     T v = alpha * acc[a][b][r];

@@ -28,6 +28,16 @@ struct hdsdp_dev {
     int cType;
     std::vector<int> heavy;  // coefficient indices handled by the dense sandwich path
     int hasZero;
+    // Gram formulations (all constraints of one kind): dense rank-one M = s s^T o (F^T W F)^2,
+    // dense M = T^T T with T_p = L^-1 A_p L^-T
+    int r1Gram, dnGram, nDn, r1Identity;
+    double *H1;              // n x (nR1 + 1): W F
+    double *Kr;              // nR1 x nR1 when the slot -> row map is not the identity
+    double *Tall;            // nDn x n^2
+    double *Dbuf, *Ubuf;     // gramBatch x n^2
+    double *Gpart;           // gramParts x nDn^2
+    int gramBatch, gramParts;
+    long gramChunk;
 
     double *S[3], *L[2], *Li[2], *LiT[2];
     long facVer[2], liVer[2];
@@ -351,6 +361,33 @@ int cg_solve(hdsdp_dev *d) {
     return 0;
 }
 
+// M(p, q) = <T_p, T_q> over the dense constraints with T_p = L^-1 A_p L^-T (so that
+// <A_p, W A_q W> = <T_p, T_q> for W = L^-T L^-1): batched GEMMs form T, a split-K
+// Gram GEMM accumulates partial blocks that are summed in a fixed order.
+void dense_gram(hdsdp_dev *d) {
+    const int n = d->n, r = d->nDn, m = d->m;
+    const long nn = (long) n * n;
+    const double *Li = d->Li[0];
+    for (int b0 = 0; b0 < r; b0 += d->gramBatch) {
+        const int cnt = std::min(d->gramBatch, r - b0);
+        hdk::unpack_dense(d->st, d->c, b0, cnt, d->Dbuf);
+        hdk::gemm<double>(d->st, false, false, false, n, n, n, 1.0, Li, n, 0, d->Dbuf, n, nn, 0.0, d->Ubuf, n, nn, cnt);
+        hdk::gemm<double>(d->st, false, true, false, n, n, n, 1.0, d->Ubuf, n, nn, Li, n, 0, 0.0,
+                          d->Tall + (size_t) b0 * nn, n, nn, cnt);
+    }
+    const long kc = d->gramChunk;
+    const int full = (int) (nn / kc);
+    const long rem = nn - (long) full * kc;
+    const long rr = (long) r * r;
+    if (full > 0)
+        hdk::gemm<double>(d->st, true, false, true, r, r, (int) kc, 1.0, d->Tall, (int) nn, kc, d->Tall, (int) nn, kc,
+                          0.0, d->Gpart, r, rr, full);
+    if (rem > 0)
+        hdk::gemm<double>(d->st, true, false, true, r, r, (int) rem, 1.0, d->Tall + (long) full * kc, (int) nn, 0,
+                          d->Tall + (long) full * kc, (int) nn, 0, 0.0, d->Gpart + (long) full * rr, r, 0, 1);
+    hdk::gram_reduce(d->st, r, full + (rem > 0 ? 1 : 0), d->Gpart, r, rr, d->c.denseCon, d->M, m);
+}
+
 }  // namespace
 
 extern "C" {
@@ -533,6 +570,27 @@ hdsdp_retcode hdsdp_dev_load(hdsdp_dev *d, const hdsdp_coeff_view *cv) {
         }
     }
 
+    // Gram paths: one kind of constraint throughout (heavy sparse rows allowed)
+    int nSmall = 0, nR1k = 0, nDnk = 0;
+    for (int k = 1; k <= m; ++k) {
+        if (smKind[k] == hdk::SM_SP || smKind[k] == hdk::SM_R1) ++nSmall;
+        if (ctype[k] == HDSDP_COEFF_DSR1) ++nR1k;
+        if (ctype[k] == HDSDP_COEFF_DENSE) ++nDnk;
+    }
+    d->nDn = nDnk;
+    d->r1Gram = (nR1k > 0 && nSmall == 0 && nDnk == 0);
+    d->dnGram = (nDnk > 0 && nSmall == 0 && nR1k == 0);
+    d->r1Identity = d->r1Gram && nR1k == m;
+    if (d->r1Gram || d->dnGram) {
+        std::vector<int> keep;
+        for (int h : d->heavy) {
+            if (d->r1Gram && ctype[h] == HDSDP_COEFF_DSR1) continue;
+            if (d->dnGram && ctype[h] == HDSDP_COEFF_DENSE) continue;
+            keep.push_back(h);
+        }
+        d->heavy.swap(keep);
+    }
+
     auto up_i = [&](int **dst, const std::vector<int> &v) -> hdsdp_retcode {
         DCALL(dalloc(d, dst, v.size()));
         if (!v.empty()) HIP_OK(hipMemcpy(*dst, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -578,6 +636,23 @@ hdsdp_retcode hdsdp_dev_load(hdsdp_dev *d, const hdsdp_coeff_view *cv) {
     DCALL(up_d(&c.smVal, smVal));
     size_t workSize = std::max((size_t) n * n, (size_t) n * std::max(c.nR1, 1));
     DCALL(dalloc(d, &d->work, workSize));
+    d->H1 = d->Kr = d->Tall = d->Dbuf = d->Ubuf = d->Gpart = nullptr;
+    if (c.nR1 > 0 || c.cDSR1) DCALL(dalloc(d, &d->H1, (size_t) n * (c.nR1 + 1)));
+    if (d->r1Gram && !d->r1Identity) DCALL(dalloc(d, &d->Kr, (size_t) c.nR1 * c.nR1));
+    if (d->dnGram) {
+        const size_t nn = (size_t) n * n;
+        const int r = d->nDn;
+        d->gramBatch = (int) std::max<size_t>(1, std::min<size_t>(r, (size_t) (512u << 20) / (nn * sizeof(double))));
+        long tiles = (long) ((r + 63) / 64) * ((r + 63) / 64 + 1) / 2;
+        d->gramParts = (int) std::min<long>(16, std::max<long>(1, 2048 / tiles));
+        long K = (long) nn;
+        d->gramChunk = std::max<long>(256, ((K + d->gramParts - 1) / d->gramParts + 15) / 16 * 16);
+        d->gramParts = (int) ((K + d->gramChunk - 1) / d->gramChunk);
+        DCALL(dalloc(d, &d->Tall, nn * r));
+        DCALL(dalloc(d, &d->Dbuf, nn * d->gramBatch));
+        DCALL(dalloc(d, &d->Ubuf, nn * d->gramBatch));
+        DCALL(dalloc(d, &d->Gpart, (size_t) r * r * d->gramParts));
+    }
     return check_launch();
 }
 
@@ -748,7 +823,8 @@ hdsdp_retcode hdsdp_dev_schur(hdsdp_dev *d, int typeKKT, double Rd, double *asin
     hdk::gemm<double>(d->st, true, false, false, n, n, n, 1.0, d->Li[0], n, 0, d->Li[0], n, 0, 0.0, d->W, n, 0, 1);
     if (Rd != 0.0)
         hdk::gemm<double>(d->st, false, false, false, n, n, n, 1.0, d->W, n, 0, d->W, n, 0, 0.0, d->W2, n, 0, 1);
-    hdk::traces(d->st, d->c, 1, m, d->W, Rd != 0.0 ? d->W2 : nullptr, Rd, d->vecA_d, d->vecB_d);
+    hdk::traces(d->st, d->c, 1, m, d->W, Rd != 0.0 ? d->W2 : nullptr, Rd, d->vecA_d, d->vecB_d, false);
+    hdk::r1_quadforms(d->st, d->c, d->W, false, 1, m, Rd, d->vecA_d, Rd != 0.0 ? d->vecB_d : nullptr, d->H1);
     *traceSinv = 0.0;
     if (typeKKT != 1 && Rd != 0.0) hdk::trace_diag(d->st, n, d->W, d->sc_d + hdk::SC_AUX);
     if (typeKKT != 1) {
@@ -756,12 +832,17 @@ hdsdp_retcode hdsdp_dev_schur(hdsdp_dev *d, int typeKKT, double Rd, double *asin
         prof_begin(d, HDSDP_PROF_PAIRS);
         hdk::schur_pairs(d->st, d->c, d->W, d->M, m);
         DCALL(prof_end(d, HDSDP_PROF_PAIRS));
+        if (d->r1Gram) {
+            if (d->r1Identity) hdk::r1_schur(d->st, d->c, d->H1, d->M, m, d->M, m);
+            else hdk::r1_schur(d->st, d->c, d->H1, d->Kr, d->c.nR1, d->M, m);
+        }
+        if (d->dnGram) dense_gram(d);
         for (int h : d->heavy) {
             // B_h = W A_h W, then M(h, k) = <A_k, B_h>
             hdk::densify(d->st, d->c, h, d->T1);
             hdk::gemm<double>(d->st, false, false, false, n, n, n, 1.0, d->W, n, 0, d->T1, n, 0, 0.0, d->T2, n, 0, 1);
             hdk::gemm<double>(d->st, false, false, false, n, n, n, 1.0, d->T2, n, 0, d->W, n, 0, 0.0, d->T1, n, 0, 1);
-            hdk::traces(d->st, d->c, 1, m, d->T1, nullptr, 0.0, d->ctmp, nullptr);
+            hdk::traces(d->st, d->c, 1, m, d->T1, nullptr, 0.0, d->ctmp, nullptr, true);
             hdk::scatter_row(d->st, m, h - 1, d->ctmp, d->M, m);
         }
         d->stats[0] += 1;
@@ -849,7 +930,8 @@ hdsdp_retcode hdsdp_dev_primal(hdsdp_dev *d, double mu, const double *y, const d
 
 hdsdp_retcode hdsdp_dev_primal_stats(hdsdp_dev *d, double *AX, double *traceCX, double *XdotS, double *traceX) {
     const int m = d->m;
-    hdk::traces(d->st, d->c, 0, m + 1, d->X, nullptr, 0.0, d->vecA_d, nullptr);
+    hdk::traces(d->st, d->c, 0, m + 1, d->X, nullptr, 0.0, d->vecA_d, nullptr, false);
+    hdk::r1_quadforms(d->st, d->c, d->X, true, 0, m + 1, 0.0, d->vecA_d, nullptr, d->H1);
     hdk::xdots(d->st, d->n, d->X, d->S[0], d->sc_d + hdk::SC_AUX);
     DCALL(check_launch());
     DCALL(download(d, d->hvecB, d->vecA_d, m + 1, d->hvecA));

@@ -44,9 +44,22 @@ void assemble_buffer(hipStream_t s, const DevCoeffs &c, const double *coef, doub
                      double eye, double *work);
 void axpy_buffer(hipStream_t s, long nn, double step, const double *src, const double *dS, double *dst);
 
-// out1[k - k0] = <A_k, B1>, out2[k - k0] = s2 <A_k, B2> for k in [k0, k0 + cnt)
+// out1[k - k0] = <A_k, B1>, out2[k - k0] = s2 <A_k, B2> for k in [k0, k0 + cnt); dense
+// rank-one coefficients are skipped (outputs untouched) unless withR1.
 void traces(hipStream_t s, const DevCoeffs &c, int k0, int cnt, const double *B1, const double *B2, double s2,
-            double *out1, double *out2);
+            double *out1, double *out2, bool withR1);
+// The same two quantities for every dense rank-one coefficient (and C when withC) by one
+// GEMM H = B F: <A_q, B> = s_q f_q^T H_q and s2 <A_q, B^2> = s2 s_q |H_q|^2.  H: n x (nR1 + 1).
+void r1_quadforms(hipStream_t s, const DevCoeffs &c, const double *B, bool withC, int k0, int cnt, double s2,
+                  double *out1, double *out2, double *H);
+// M entries between dense rank-one constraints: K = F^T H (H = W F from r1_quadforms),
+// M = s s^T o K o K.  K: nR1 x nR1 scratch (ldk).
+void r1_schur(hipStream_t s, const DevCoeffs &c, const double *H, double *K, long ldk, double *M, long ldm);
+// full n x n copies of dense slots slot0 .. slot0 + cnt - 1
+void unpack_dense(hipStream_t s, const DevCoeffs &c, int slot0, int cnt, double *out);
+// M(denseCon[p] - 1, denseCon[q] - 1) = sum_b P_b(p, q), p >= q
+void gram_reduce(hipStream_t s, int r, int nPart, const double *P, long ldp, long strideP, const int *denseCon,
+                 double *M, long ldm);
 // M_ij = <A_i, W A_j W> for pairs of small constraints (lower triangle)
 void schur_pairs(hipStream_t s, const DevCoeffs &c, const double *W, double *M, long ldm);
 void scatter_row(hipStream_t s, int m, int h, const double *t, double *M, long ldm);

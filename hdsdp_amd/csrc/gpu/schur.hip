@@ -108,11 +108,12 @@ __device__ inline double wave_sum(double v) {
 // out1[k] = <A_k, B1>, out2[k] = s2 <A_k, B2> (B2 may be NULL) for coefficients k0 .. k0 + cnt - 1,
 // written to out[k - k0].  One workgroup of 256 threads per coefficient.
 __global__ __launch_bounds__(256) void trace_kernel(int n, DevCoeffs c, int k0, const double *B1, const double *B2,
-                                                    double s2, double *out1, double *out2) {
+                                                    double s2, double *out1, double *out2, int withR1) {
     __shared__ double red1[4], red2[4];
     const int k = k0 + blockIdx.x;
     const int tid = threadIdx.x;
     const int t = c.ctype[k];
+    if (t == COEFF_DSR1 && !withR1) return;  // left to r1_quadforms
     double a1 = 0.0, a2 = 0.0;
     if (t == COEFF_SPARSE) {
         for (int e = c.cBeg[k] + tid; e < c.cBeg[k + 1]; e += 256) {
@@ -177,9 +178,104 @@ __global__ __launch_bounds__(256) void trace_kernel(int n, DevCoeffs c, int k0, 
 }
 
 void traces(hipStream_t s, const DevCoeffs &c, int k0, int cnt, const double *B1, const double *B2, double s2,
-            double *out1, double *out2) {
+            double *out1, double *out2, bool withR1) {
     if (cnt <= 0) return;
-    trace_kernel<<<cnt, 256, 0, s>>>(c.n, c, k0, B1, B2, s2, out1, out2);
+    trace_kernel<<<cnt, 256, 0, s>>>(c.n, c, k0, B1, B2, s2, out1, out2, withR1 ? 1 : 0);
+}
+
+// ---- dense rank-one coefficients through GEMMs ------------------------------------------
+
+// One wave per factor column q: with h = H[:, q] = B f_q,
+//   out1[con - k0] = sign f_q^T h,  out2[con - k0] = s2 sign h^T h.
+// Columns 0 .. nR1 - 1 are the DSR1 constraints (coefficient rCon[q]); column nR1,
+// when present, is a dense rank-one C (coefficient 0).
+__global__ __launch_bounds__(256) void r1_vectors_kernel(int n, int nR1, int nCols, const int *rCon,
+                                                         const double *csign, const double *F, const double *H,
+                                                         int k0, int cnt, double s2, double *out1, double *out2) {
+    const int q = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (q >= nCols) return;
+    const int con = (q < nR1) ? rCon[q] : 0;
+    const double *f = F + (long) q * n, *h = H + (long) q * n;
+    double d1 = 0.0, d2 = 0.0;
+    for (int i = lane; i < n; i += 64) {
+        d1 += f[i] * h[i];
+        d2 += h[i] * h[i];
+    }
+    d1 = wave_sum(d1);
+    d2 = wave_sum(d2);
+    if (lane == 0 && con >= k0 && con < k0 + cnt) {
+        out1[con - k0] = csign[con] * d1;
+        if (out2) out2[con - k0] = s2 * csign[con] * d2;
+    }
+}
+
+void r1_quadforms(hipStream_t s, const DevCoeffs &c, const double *B, bool withC, int k0, int cnt, double s2,
+                  double *out1, double *out2, double *H) {
+    const int n = c.n;
+    const int nCols = c.nR1 + ((withC && c.cDSR1) ? 1 : 0);
+    if (nCols == 0) return;
+    gemm<double>(s, false, false, false, n, nCols, n, 1.0, B, n, 0, c.rF, n, 0, 0.0, H, n, 0, 1);
+    r1_vectors_kernel<<<(nCols + 3) / 4, 256, 0, s>>>(n, c.nR1, nCols, c.rCon, c.csign, c.rF, H, k0, cnt, s2,
+                                                      out1, out2);
+}
+
+// M(rp, rq) = s_p s_q K_pq^2 for p >= q over the DSR1 slots, rp = rCon[p] - 1 (lower triangle of M).
+__global__ void r1_square_kernel(int nR1, const int *rCon, const double *csign, const double *K, long ldk,
+                                 double *M, long ldm) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = blockIdx.y;
+    if (p >= nR1 || p < q) return;
+    const int cp = rCon[p], cq = rCon[q];
+    const double kv = K[p + (long) q * ldk];
+    const long i = (cp > cq ? cp : cq) - 1, j = (cp > cq ? cq : cp) - 1;
+    M[i + j * ldm] = csign[cp] * csign[cq] * kv * kv;
+}
+
+void r1_schur(hipStream_t s, const DevCoeffs &c, const double *H, double *K, long ldk, double *M, long ldm) {
+    const int n = c.n, r = c.nR1;
+    if (r == 0) return;
+    // K = F^T (W F), lower triangle
+    gemm<double>(s, true, false, true, r, r, n, 1.0, c.rF, n, 0, H, n, 0, 0.0, K, (int) ldk, 0, 1);
+    r1_square_kernel<<<dim3((r + 255) / 256, r), 256, 0, s>>>(r, c.rCon, c.csign, K, ldk, M, ldm);
+}
+
+// ---- dense coefficients through a Gram matrix -------------------------------------------
+
+// out[b] = full n x n matrix of dense slot slot0 + b (unpacked from the lower packed column)
+__global__ void unpack_dense_kernel(int n, const double *dVal, int slot0, double *out) {
+    const long npk = (long) n * (n + 1) / 2;
+    const double *v = dVal + (long) (slot0 + blockIdx.y) * npk;
+    double *o = out + (long) blockIdx.y * n * n;
+    for (long e = blockIdx.x * (long) blockDim.x + threadIdx.x; e < (long) n * n; e += (long) gridDim.x * blockDim.x) {
+        long i = e % n, j = e / n;
+        long r = i > j ? i : j, q = i > j ? j : i;
+        o[e] = v[q * n - q * (q + 1) / 2 + r];
+    }
+}
+
+void unpack_dense(hipStream_t s, const DevCoeffs &c, int slot0, int cnt, double *out) {
+    if (cnt <= 0) return;
+    unpack_dense_kernel<<<dim3(256, cnt), 256, 0, s>>>(c.n, c.dVal, slot0, out);
+}
+
+// M(i, j) (lower) = sum over the partial Gram blocks, in block order; i, j index dense slots,
+// mapped to constraints through denseCon.
+__global__ void gram_reduce_kernel(int r, int nPart, const double *P, long ldp, long strideP, const int *denseCon,
+                                   double *M, long ldm) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = blockIdx.y;
+    if (p >= r || p < q) return;
+    double v = 0.0;
+    for (int b = 0; b < nPart; ++b) v += P[(long) b * strideP + p + (long) q * ldp];
+    const int cp = denseCon[p], cq = denseCon[q];
+    const long i = (cp > cq ? cp : cq) - 1, j = (cp > cq ? cq : cp) - 1;
+    M[i + j * ldm] = v;
+}
+
+void gram_reduce(hipStream_t s, int r, int nPart, const double *P, long ldp, long strideP, const int *denseCon,
+                 double *M, long ldm) {
+    gram_reduce_kernel<<<dim3((r + 255) / 256, r), 256, 0, s>>>(r, nPart, P, ldp, strideP, denseCon, M, ldm);
 }
 
 // ---- Schur pair kernel over small sparse / rank-one constraints -----------------------
@@ -309,6 +405,21 @@ __global__ void densify_kernel(int n, DevCoeffs c, int k, double *out) {
                 int p = e % nnz, q = e / nnz;
                 out[c.cRow[b + p] + (long) c.cRow[b + q] * n] = c.csign[k] * c.cVal[b + p] * c.cVal[b + q];
             }
+        }
+    } else if (t == COEFF_DENSE) {
+        const long npk = (long) n * (n + 1) / 2;
+        const double *v = c.dVal + (long) c.dSlot[k] * npk;
+        for (long e = threadIdx.x + blockIdx.x * (long) blockDim.x; e < (long) n * n; e += (long) blockDim.x * gridDim.x) {
+            long i = e % n, j = e / n;
+            long r = i > j ? i : j, q = i > j ? j : i;
+            out[e] = v[q * n - q * (q + 1) / 2 + r];
+        }
+    } else if (t == COEFF_DSR1) {
+        const double *f = c.rF + (long) c.rSlot[k] * n;
+        const double sg = c.csign[k];
+        for (long e = threadIdx.x + blockIdx.x * (long) blockDim.x; e < (long) n * n; e += (long) blockDim.x * gridDim.x) {
+            long i = e % n, j = e / n;
+            out[e] = sg * f[i] * f[j];
         }
     }
 }
