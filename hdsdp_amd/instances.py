@@ -185,11 +185,13 @@ def dense_random(n: int, m: int, seed: int) -> SDPInstance:
     return SDPInstance(f"dense_n{n}_m{m}_s{seed}", n, m, beg, idx, elem, b)
 
 
-def rank_one(n: int, m: int, r: int, seed: int) -> SDPInstance:
+def rank_one(n: int, m: int, r: int, seed: int, form: str = "factor") -> SDPInstance:
     """Rank-one constraints ``A_k = a_k a_k^T``, ``a_k ~ N(0, I/n)``;
     ``b_k = a_k^T X0 a_k`` with ``X0 = Z Z^T / r + 1e-3 I``; ``C = I``.
 
-    Returned in factor form (``factor``/``sign``); ``C`` is the only CSC column.
+    ``form="factor"`` returns the factor form (``factor``/``sign``, ``C`` the only
+    CSC column); ``form="csc"`` expands every ``A_k`` into packed-lower CSC, the
+    reference's input format (n(n+1)/2 entries per constraint).
     """
     rng = np.random.default_rng(seed)
     a = rng.standard_normal((n, m)) / np.sqrt(n)
@@ -197,9 +199,22 @@ def rank_one(n: int, m: int, r: int, seed: int) -> SDPInstance:
     za = z.T @ a  # r x m
     b = (za * za).sum(axis=0) / r + 1e-3 * (a * a).sum(axis=0)
     diag = packed_index(n, np.arange(n), np.arange(n))
+    name = f"rankone_n{n}_m{m}_s{seed}"
+    if form == "csc":
+        il, jl = np.tril_indices(n)  # row i >= column j
+        pk = packed_index(n, il, jl)
+        order = np.argsort(pk)
+        il, jl, pk = il[order], jl[order], pk[order]
+        cols = [(diag, np.ones(n))]
+        for k in range(m):
+            cols.append((pk, a[il, k] * a[jl, k]))
+        inst = _assemble(name, n, cols, b)
+        return inst
+    if form != "factor":
+        raise ValueError(form)
     beg = np.zeros(m + 2, dtype=np.int64)
     beg[1:] = n
-    inst = SDPInstance(f"rankone_n{n}_m{m}_s{seed}", n, m, beg, diag.astype(np.int32),
+    inst = SDPInstance(name, n, m, beg, diag.astype(np.int32),
                        np.ones(n), b, factor=np.asfortranarray(a), sign=np.ones(m))
     return inst
 

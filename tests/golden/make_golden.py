@@ -38,6 +38,9 @@ CASES = {
     "maxcut_n60": (lambda: I.maxcut(60, 0.2, 22, "unif"), {"relopt": 1e-6, "absopt": 1e30}),
     "dense_n20_m30": (lambda: I.dense_random(20, 30, 23), {"relopt": 1e-6, "absopt": 1e30}),
     "dense_n40_m60": (lambda: I.dense_random(40, 60, 24), {"relopt": 1e-6, "absopt": 1e30}),
+    # the dense (C3) and rank-one (C4) families at sizes the CPU reference solves in seconds
+    "dense_n120_m240": (lambda: I.dense_random(120, 240, 33), {"relopt": 1e-6, "absopt": 1e30}),
+    "rankone_n60_m500": (lambda: I.rank_one(60, 500, 5, 45, form="csc"), {"relopt": 1e-6, "absopt": 1e30}),
 }
 
 LOG_RE = re.compile(r"^\s+(\d+)\s+([+-]\d\.\d+e[+-]\d+)\s+([+-]\d\.\d+e[+-]\d+)\s+(\S+)\s+(\S+)\s+(\S+)\s+(\S+)\s+(\S+)")

@@ -32,6 +32,8 @@ CASES = {
     "c1_gap1e-6": (1e-7, 1e-5, False),
     "c2_gap1e-6": (1e-7, 1e-5, False),
     "c5_gap1e-6": (1e-7, 1e-5, False),
+    "dense_n120_m240": (1e-7, 1e-5, False),
+    "rankone_n60_m500": (1e-7, 1e-5, False),
 }
 
 
@@ -59,6 +61,17 @@ def test_solve_matches_reference(gpu_lib, oracle, name):
         assert ores.iters == res.iters and ores.status == res.status
         assert abs(ores.dobj - res.dobj) <= 1e-8 * max(1.0, abs(res.dobj))
         assert abs(ores.pobj - res.pobj) <= ptol * max(1.0, abs(res.pobj))
+
+
+def test_rank_one_factor_form(gpu_lib):
+    """The factor-form input (A_k = s_k a_k a_k^T handed over without a CSC
+    expansion) reaches the reference's result for the same instance."""
+    from hdsdp_amd import instances as I
+    from hdsdp_amd.solver import run_solver
+
+    _, fix = dt.load_golden("rankone_n60_m500")
+    res = run_solver(gpu_lib, I.rank_one(60, 500, 5, 45), relopt=1e-6, absopt=1e30)
+    check_against_golden(fix, res, 1e-7, 1e-5)
 
 
 def test_solution_checks(gpu_lib):

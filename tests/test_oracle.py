@@ -187,6 +187,7 @@ KNOWN = {
     "theta_petersen": (lambda: I.theta_graph(10, *_petersen()), -4.0),
     "theta_K7": (lambda: I.theta_graph(7, *_complete(7)), -1.0),
     "theta_C7": (lambda: I.theta_graph(7, *_cycle(7)), -7 * np.cos(np.pi / 7) / (1 + np.cos(np.pi / 7))),
+    "theta_empty6": (lambda: I.theta_graph(6, [], []), -6.0),
     # MaxCut relaxation value n/2 (1 - cos(pi (n-1)/n)) on odd cycles, n^2/4 on K_n
     "maxcut_C5": (lambda: I.maxcut_graph(5, *_cycle(5), np.ones(5)), -(25 + 5 * np.sqrt(5.0)) / 8),
     "maxcut_K6": (lambda: I.maxcut_graph(6, *_complete(6), np.ones(15)), -9.0),
@@ -212,6 +213,8 @@ GOLDEN_CASES = {
     "dense_n40_m60": (1e-8, 1e-6),
     "c1_default": (1e-8, 1e-7),
     "c1_gap1e-6": (1e-7, 1e-5),
+    "dense_n120_m240": (1e-7, 1e-5),
+    "rankone_n60_m500": (1e-7, 1e-5),
 }
 
 
