@@ -36,13 +36,25 @@ struct Mfma<float> {
     }
 };
 
+// k range of one output tile when an operand is triangular: tri 1 = op(B) lower
+// (B(k, j) = 0 for k < j), tri 2 = op(A) lower (A(i, k) = 0 for k > i).  Only exact
+// zeros are skipped, so the result is the same as the full product.
+__device__ inline void tri_krange(int tri, int i0, int j0, int tile, int K, int &kLo, int &kHi) {
+    kLo = 0;
+    kHi = K;
+    if (tri == 1) kLo = (j0 / KSTEP) * KSTEP;
+    if (tri == 2) kHi = (i0 + tile < K) ? i0 + tile : K;
+}
+
 template <typename T, bool TA, bool TB, bool LOWER>
 __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha, const T *A, int lda,
                                                    long sA, const T *B, int ldb, long sB, T beta,
-                                                   T *C, int ldc, long sC) {
+                                                   T *C, int ldc, long sC, int tri) {
     typedef typename Mfma<T>::acc_t acc_t;
     const int i0 = blockIdx.x * TILE, j0 = blockIdx.y * TILE;
     if (LOWER && i0 + TILE - 1 < j0) return;
+    int kLo, kHi;
+    tri_krange(tri, i0, j0, TILE, K, kLo, kHi);
     A += (long) blockIdx.z * sA;
     B += (long) blockIdx.z * sB;
     C += (long) blockIdx.z * sC;
@@ -55,7 +67,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha,
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b) acc[a][b] = (acc_t){0, 0, 0, 0};
 
-    for (int k0 = 0; k0 < K; k0 += KSTEP) {
+    for (int k0 = kLo; k0 < kHi; k0 += KSTEP) {
 #pragma unroll
         for (int e = tid; e < TILE * KSTEP; e += 256) {
             int i, k;
@@ -86,32 +98,184 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha,
         }
         __syncthreads();
     }
+    T cin[2][2][4];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                int i = i0 + wi + a * 16 + Mfma<T>::row(lk, r);
-                int j = j0 + wj + b * 16 + lr;
-                if (i < M && j < N && (!LOWER || i >= j)) {
-                    T v = alpha * acc[a][b][r];
-                    if (beta != (T) 0) v += beta * C[i + (long) j * ldc];
-                    C[i + (long) j * ldc] = v;
-                }
+                const int i = i0 + wi + a * 16 + Mfma<T>::row(lk, r);
+                const int j = j0 + wj + b * 16 + lr;
+                const bool ok = i < M && j < N && (!LOWER || i >= j);
+                cin[a][b][r] = (ok && beta != (T) 0) ? C[i + (long) j * ldc] : (T) 0;
             }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = i0 + wi + a * 16 + Mfma<T>::row(lk, r);
+                const int j = j0 + wj + b * 16 + lr;
+                if (i < M && j < N && (!LOWER || i >= j)) C[i + (long) j * ldc] = alpha * acc[a][b][r] + beta * cin[a][b][r];
+            }
+}
+
+// 128 x 128 tiles for large products: four waves, each a 64 x 64 quadrant as 4 x 4
+// MFMA 16x16 accumulators; k slabs of 16 double-buffered in LDS so the global
+// loads of slab s + 1 are in flight while slab s is multiplied (one barrier per slab).
+constexpr int BT = 128, BPAD = 16;
+static_assert(4 * 16 * 65 <= 2 * KSTEP * (BT + BPAD), "epilogue staging must fit in the k-slab buffers");
+
+template <typename T, bool TA, bool TB, bool CHECK>
+__device__ inline void big_load(const T *A, int lda, const T *B, int ldb, int M, int N, int K, int i0, int j0,
+                                int k0, int tid, T (&ra)[8], T (&rb)[8]) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        int i, k;
+        if (!TA) { k = tid >> 4; i = (tid & 15) * 8 + u; } else { i = tid >> 1; k = (tid & 1) * 8 + u; }
+        const int gi = i0 + i, gk = k0 + k;
+        const T *pa = TA ? A + gk + (long) gi * lda : A + gi + (long) gk * lda;
+        ra[u] = (!CHECK || (gi < M && gk < K)) ? *pa : (T) 0;
+        int j, kb;
+        if (TB) { kb = tid >> 4; j = (tid & 15) * 8 + u; } else { j = tid >> 1; kb = (tid & 1) * 8 + u; }
+        const int gj = j0 + j, gkb = k0 + kb;
+        const T *pb = TB ? B + gj + (long) gkb * ldb : B + gkb + (long) gj * ldb;
+        rb[u] = (!CHECK || (gj < N && gkb < K)) ? *pb : (T) 0;
+    }
+}
+
+// one k slab of the tile into registers; the bounds checks are dropped for interior slabs
+template <typename T, bool TA, bool TB>
+__device__ inline void big_fetch(const T *A, int lda, const T *B, int ldb, int M, int N, int K, int i0, int j0,
+                                 int k0, int tid, T (&ra)[8], T (&rb)[8]) {
+    if (i0 + BT <= M && j0 + BT <= N && k0 + KSTEP <= K)
+        big_load<T, TA, TB, false>(A, lda, B, ldb, M, N, K, i0, j0, k0, tid, ra, rb);
+    else
+        big_load<T, TA, TB, true>(A, lda, B, ldb, M, N, K, i0, j0, k0, tid, ra, rb);
+}
+
+template <typename T, bool TA, bool TB>
+__device__ inline void big_store(T (*As)[BT + BPAD], T (*Bs)[BT + BPAD], int tid, const T (&ra)[8],
+                                 const T (&rb)[8]) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        int i, k;
+        if (!TA) { k = tid >> 4; i = (tid & 15) * 8 + u; } else { i = tid >> 1; k = (tid & 1) * 8 + u; }
+        As[k][i] = ra[u];
+        int j, kb;
+        if (TB) { kb = tid >> 4; j = (tid & 15) * 8 + u; } else { j = tid >> 1; kb = (tid & 1) * 8 + u; }
+        Bs[kb][j] = rb[u];
+    }
+}
+
+template <typename T, bool TA, bool TB, bool LOWER>
+__global__ __launch_bounds__(256) void gemm_big_kernel(int M, int N, int K, T alpha, const T *A, int lda, long sA,
+                                                       const T *B, int ldb, long sB, T beta, T *C, int ldc,
+                                                       long sC, int tri) {
+    typedef typename Mfma<T>::acc_t acc_t;
+    const int i0 = blockIdx.x * BT, j0 = blockIdx.y * BT;
+    if (LOWER && i0 + BT - 1 < j0) return;
+    int kLo, kHi;
+    tri_krange(tri, i0, j0, BT, K, kLo, kHi);
+    A += (long) blockIdx.z * sA;
+    B += (long) blockIdx.z * sB;
+    C += (long) blockIdx.z * sC;
+    __shared__ T As[2][KSTEP][BT + BPAD];
+    __shared__ T Bs[2][KSTEP][BT + BPAD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wi = (wave & 1) * 64, wj = (wave >> 1) * 64;
+    const int lr = lane & 15, lk = lane >> 4;
+    acc_t acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = (acc_t){0, 0, 0, 0};
+    T ra[8], rb[8];
+    big_fetch<T, TA, TB>(A, lda, B, ldb, M, N, K, i0, j0, kLo, tid, ra, rb);
+    big_store<T, TA, TB>(As[0], Bs[0], tid, ra, rb);
+    __syncthreads();
+    const int nslab = (kHi - kLo + KSTEP - 1) / KSTEP;
+    for (int sl = 0; sl < nslab; ++sl) {
+        const int cur = sl & 1;
+        const bool more = sl + 1 < nslab;
+        if (more) big_fetch<T, TA, TB>(A, lda, B, ldb, M, N, K, i0, j0, kLo + (sl + 1) * KSTEP, tid, ra, rb);
+#pragma unroll
+        for (int kk = 0; kk < KSTEP; kk += 4) {
+            T af[4], bf[4];
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                af[f] = As[cur][kk + lk][wi + 16 * f + lr];
+                bf[f] = Bs[cur][kk + lk][wj + 16 * f + lr];
+            }
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[a][b] = Mfma<T>::op(af[a], bf[b], acc[a][b]);
+        }
+        if (more) big_store<T, TA, TB>(As[cur ^ 1], Bs[cur ^ 1], tid, ra, rb);
+        __syncthreads();
+    }
+    // Epilogue: each wave stages its 64 x 64 quadrant through LDS (the k-slab buffers,
+    // free after the last barrier) 16 columns at a time, so that lane q reads and
+    // writes row q of C: 64 consecutive rows per column, coalesced.
+    T (*stage)[65] = reinterpret_cast<T (*)[65]>(&As[0][0][0]) + wave * 16;
+    const int i = i0 + wi + lane;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) stage[lr][a * 16 + Mfma<T>::row(lk, r)] = acc[a][b][r];
+        // all loads of the strip are issued before any store (C may alias nothing,
+        // but the compiler cannot know that)
+        T cin[16];
+#pragma unroll
+        for (int cc = 0; cc < 16; ++cc) {
+            const int j = j0 + wj + b * 16 + cc;
+            const bool ok = i < M && j < N && (!LOWER || i >= j);
+            cin[cc] = (ok && beta != (T) 0) ? C[i + (long) j * ldc] : (T) 0;
+        }
+#pragma unroll
+        for (int cc = 0; cc < 16; ++cc) {
+            const int j = j0 + wj + b * 16 + cc;
+            if (i < M && j < N && (!LOWER || i >= j)) C[i + (long) j * ldc] = alpha * stage[cc][lane] + beta * cin[cc];
+        }
+    }
 }
 
 template <typename T>
 void gemm(hipStream_t s, bool transA, bool transB, bool lower, int M, int N, int K, T alpha, const T *A,
           int lda, long strideA, const T *B, int ldb, long strideB, T beta, T *C, int ldc, long strideC,
-          int batch) {
+          int batch, int tri) {
     if (M <= 0 || N <= 0 || batch <= 0) return;
+    const long bigTiles = (long) ((M + BT - 1) / BT) * ((N + BT - 1) / BT) * batch / (lower ? 2 : 1);
+    // the 128-wide tiles pay off only when the k loop is long enough to hide their
+    // prologue and epilogue (measured: rank-64 updates run faster on 64-wide tiles)
+    if (bigTiles >= 192 && K >= 256) {
+        dim3 bgrid((M + BT - 1) / BT, (N + BT - 1) / BT, batch);
+#define HDK_GEMM_BIG(ta, tb, lo)                                                                          \
+        if (transA == ta && transB == tb && lower == lo) {                                                \
+            gemm_big_kernel<T, ta, tb, lo><<<bgrid, 256, 0, s>>>(M, N, K, alpha, A, lda, strideA, B, ldb, \
+                                                                 strideB, beta, C, ldc, strideC, tri);    \
+            return;                                                                                       \
+        }
+        HDK_GEMM_BIG(false, false, false)
+        HDK_GEMM_BIG(false, true, false)
+        HDK_GEMM_BIG(true, false, false)
+        HDK_GEMM_BIG(true, true, false)
+        HDK_GEMM_BIG(false, false, true)
+        HDK_GEMM_BIG(false, true, true)
+        HDK_GEMM_BIG(true, false, true)
+        HDK_GEMM_BIG(true, true, true)
+#undef HDK_GEMM_BIG
+    }
     dim3 grid((M + TILE - 1) / TILE, (N + TILE - 1) / TILE, batch);
 #define HDK_GEMM_CASE(ta, tb, lo)                                                                    \
     if (transA == ta && transB == tb && lower == lo) {                                               \
         gemm_kernel<T, ta, tb, lo><<<grid, 256, 0, s>>>(M, N, K, alpha, A, lda, strideA, B, ldb, strideB, \
-                                                        beta, C, ldc, strideC);                     \
+                                                        beta, C, ldc, strideC, tri);                \
         return;                                                                                      \
     }
     HDK_GEMM_CASE(false, false, false)
@@ -126,116 +290,164 @@ void gemm(hipStream_t s, bool transA, bool transB, bool lower, int M, int N, int
 }
 
 template void gemm<double>(hipStream_t, bool, bool, bool, int, int, int, double, const double *, int, long,
-                           const double *, int, long, double, double *, int, long, int);
+                           const double *, int, long, double, double *, int, long, int, int);
 template void gemm<float>(hipStream_t, bool, bool, bool, int, int, int, float, const float *, int, long,
-                          const float *, int, long, float, float *, int, long, int);
+                          const float *, int, long, float, float *, int, long, int, int);
 
-// ---- Cholesky --------------------------------------------------------------
+// ---- Cholesky ----------------------------------------------------------------
 
-// Factor the kb x kb diagonal block at A (lower) in LDS and write the inverse of
-// its factor to Dinv (NB x NB, lower, zeros above).  Sets *info = k0 + j + 1 at
-// the first non-positive pivot.
+// Broadcast the value that lane `src` holds (src uniform) to the whole wave.
+__device__ inline double lane_bcast(double v, int src) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int) (b & 0xffffffffLL), src);
+    const int hi = __builtin_amdgcn_readlane((int) (b >> 32), src);
+    return __longlong_as_double(((long long) (unsigned) lo) | ((long long) hi << 32));
+}
+__device__ inline float lane_bcast(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+
+// Column r of L^-1 for the 64 x 64 lower factor whose row q is held by lane q in
+// a[0..q] (forward substitution L x = e_r, rows of L broadcast by readlane).
 template <typename T>
-__global__ __launch_bounds__(256) void potf2_kernel(T *A, int lda, int k0, int kb, int *info, T *Dinv) {
-    __shared__ T L[NB][NB + 1];
-    __shared__ T X[NB][NB + 1];
-    __shared__ int fail;
+__device__ inline void leaf_inverse(const T (&a)[NB], int r, T (&x)[NB]) {
+#pragma unroll
+    for (int j = 0; j < NB; ++j) x[j] = (j == r) ? (T) 1 : (T) 0;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        x[j] = x[j] / lane_bcast(a[j], j);
+#pragma unroll
+        for (int k = j + 1; k < NB; ++k) x[k] -= lane_bcast(a[j], k) * x[j];
+    }
+}
+
+// One wave factors a kb x kb diagonal block (kb <= 64) and inverts its factor.
+// Lane r holds row r in registers; the column needed at step j is broadcast lane by
+// lane (readlane), so the whole factorisation runs without LDS or barriers:
+//   step j: p = a(j, j); row r: t = a(r, j) / p; a(r, c) -= t a(c, j) for c > j;
+//           then column j is scaled by 1 / sqrt(p) and a(j, j) = sqrt(p).
+// Entries above the diagonal are scratch.  The inverse X = L^-1 follows by forward
+// substitution, lane c computing column c.  Rows and columns beyond kb are the
+// identity.  Writes L (lower) in place and X into Dinv (64 x 64, zeros above);
+// sets *info = k0 + j + 1 at the first non-positive pivot.
+template <typename T>
+__global__ __launch_bounds__(64) void chol_leaf_kernel(T *A, int lda, int k0, int kb, int *info, T *Dinv, int ldd) {
     if (*info) return;
-    const int tid = threadIdx.x;
-    for (int e = tid; e < kb * kb; e += 256) {
-        int r = e % kb, c = e / kb;
-        L[r][c] = (r >= c) ? A[r + (long) c * lda] : (T) 0;
+    const int r = threadIdx.x;
+    T a[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        T v = (r == c) ? (T) 1 : (T) 0;
+        if (r < kb && c < kb && r >= c) v = A[r + (long) c * lda];
+        a[c] = v;
     }
-    if (tid == 0) fail = 0;
-    __syncthreads();
-    for (int j = 0; j < kb; ++j) {
-        if (tid == 0) {
-            T d = L[j][j];
-            if (!(d > (T) 0)) fail = j + 1;
-            else L[j][j] = sqrt(d);
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const T p = lane_bcast(a[j], j);
+        if (!(p > (T) 0)) {
+            if (r == 0) *info = k0 + j + 1;
+            return;
         }
-        __syncthreads();
-        if (fail) break;
-        T inv = (T) 1 / L[j][j];
-        for (int r = j + 1 + tid; r < kb; r += 256) L[r][j] *= inv;
-        __syncthreads();
-        int rem = kb - j - 1;
-        for (int e = tid; e < rem * rem; e += 256) {
-            int r = j + 1 + e % rem, c = j + 1 + e / rem;
-            if (r >= c) L[r][c] -= L[r][j] * L[c][j];
-        }
-        __syncthreads();
+        const T t = a[j] / p;
+#pragma unroll
+        for (int c = j + 1; c < NB; ++c) a[c] -= t * lane_bcast(a[j], c);
+        const T sq = sqrt(p);
+        a[j] = (r == j) ? sq : a[j] / sq;
     }
-    if (fail) {
-        if (tid == 0) *info = k0 + fail;
+    T x[NB];
+    leaf_inverse<T>(a, r, x);
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+        if (r < kb && c < kb && r >= c) A[r + (long) c * lda] = a[c];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) Dinv[k + (long) r * ldd] = (k >= r) ? x[k] : (T) 0;
+}
+
+// Right-looking blocked Cholesky.  Per block column k (width NB):
+//   leaf:  factor the diagonal block and invert its factor (work)
+//   panel: P = P L11^-T as a GEMM with that inverse
+//   trail: A22 -= P P^T (lower)
+// With look-ahead the trailing update is split: the next block column's part runs on
+// s (so that the next leaf and panel can start at once), the rest on s2; the next
+// block column's own update waits for s2's previous update through an event.
+template <typename T>
+void potrf(hipStream_t s, int n, T *A, int lda, T *work, int *info, const LookAhead *la) {
+    hipMemsetAsync(info, 0, sizeof(int), s);
+    auto at = [&](int i, int j) { return A + i + (long) j * lda; };
+    auto leaf = [&](int k0) {
+        const int kb = (n - k0 < NB) ? n - k0 : NB;
+        chol_leaf_kernel<T><<<1, 64, 0, s>>>(at(k0, k0), lda, k0, kb, info, work, NB);
+        if (k0 + kb < n)
+            gemm<T>(s, false, true, false, n - k0 - kb, kb, kb, (T) 1, at(k0 + kb, k0), lda, 0, work, NB, 0, (T) 0,
+                    at(k0 + kb, k0), lda, 0, 1);
+    };
+    if (!la || n <= 4 * NB) {
+        for (int k0 = 0; k0 < n; k0 += NB) {
+            leaf(k0);
+            const int k1 = k0 + NB;
+            if (k1 < n)
+                gemm<T>(s, false, true, true, n - k1, n - k1, NB, (T) -1, at(k1, k0), lda, 0, at(k1, k0), lda, 0,
+                        (T) 1, at(k1, k1), lda, 0, 1);
+        }
         return;
     }
-    // inverse of the block factor, one column per thread
-    if (tid < kb) {
-        int c = tid;
-        X[c][c] = (T) 1 / L[c][c];
-        for (int r = c + 1; r < kb; ++r) {
-            T sum = 0;
-            for (int l = c; l < r; ++l) sum += L[r][l] * X[l][c];
-            X[r][c] = -sum / L[r][r];
+    int ring = 0;
+    bool pending = false;  // an update of s2 that the next block column must wait for
+    hipEvent_t evB = nullptr;
+    leaf(0);
+    for (int k0 = 0;; k0 += NB) {
+        const int k1 = k0 + NB;
+        if (k1 >= n) break;
+        const int nb1 = (n - k1 < NB) ? n - k1 : NB, k2 = k1 + nb1;
+        hipEvent_t evBprev = evB;
+        const bool waitPrev = pending;
+        if (k2 < n) {
+            // the rest of the trailing matrix on s2, as soon as this block column's panel is done
+            hipEvent_t evP = la->ev[ring++ % LA_EVENTS];
+            hipEventRecord(evP, s);
+            hipStreamWaitEvent(la->s2, evP, 0);
+            gemm<T>(la->s2, false, true, true, n - k2, n - k2, NB, (T) -1, at(k2, k0), lda, 0, at(k2, k0), lda, 0,
+                    (T) 1, at(k2, k2), lda, 0, 1);
+            evB = la->ev[ring++ % LA_EVENTS];
+            hipEventRecord(evB, la->s2);
+            pending = true;
+        } else {
+            pending = false;
         }
+        // the next block column [k1, k2): its last earlier update ran on s2
+        if (waitPrev) hipStreamWaitEvent(s, evBprev, 0);
+        gemm<T>(s, false, true, true, n - k1, nb1, NB, (T) -1, at(k1, k0), lda, 0, at(k1, k0), lda, 0, (T) 1,
+                at(k1, k1), lda, 0, 1);
+        leaf(k1);
     }
-    __syncthreads();
-    for (int e = tid; e < kb * kb; e += 256) {
-        int r = e % kb, c = e / kb;
-        if (r >= c) A[r + (long) c * lda] = L[r][c];
-        Dinv[r + c * NB] = (r >= c) ? X[r][c] : (T) 0;
-    }
+    if (pending) hipStreamWaitEvent(s, evB, 0);
 }
 
-template <typename T>
-void potrf(hipStream_t s, int n, T *A, int lda, T *work, int *info) {
-    hipMemsetAsync(info, 0, sizeof(int), s);
-    for (int k0 = 0; k0 < n; k0 += NB) {
-        int kb = (n - k0 < NB) ? n - k0 : NB;
-        potf2_kernel<T><<<1, 256, 0, s>>>(A + k0 + (long) k0 * lda, lda, k0, kb, info, work);
-        int k1 = k0 + kb;
-        if (k1 < n) {
-            T *P = A + k1 + (long) k0 * lda;
-            // panel: P = P * Dinv^T (each workgroup owns its rows, so in place is safe)
-            gemm<T>(s, false, true, false, n - k1, kb, kb, (T) 1, P, lda, 0, work, NB, 0, (T) 0, P, lda, 0, 1);
-            // trailing: A22 -= P P^T (lower)
-            gemm<T>(s, false, true, true, n - k1, n - k1, kb, (T) -1, P, lda, 0, P, lda, 0, (T) 1,
-                    A + k1 + (long) k1 * lda, lda, 0, 1);
-        }
-    }
-}
-
-template void potrf<double>(hipStream_t, int, double *, int, double *, int *);
-template void potrf<float>(hipStream_t, int, float *, int, float *, int *);
+template void potrf<double>(hipStream_t, int, double *, int, double *, int *, const LookAhead *);
+template void potrf<float>(hipStream_t, int, float *, int, float *, int *, const LookAhead *);
 
 // ---- triangular inverse ------------------------------------------------------
 
+// Inverse of each 64 x 64 diagonal block of the lower factor (the last one may be
+// smaller, padded with the identity): one wave per block, lane q loads row q.
 template <typename T>
-__global__ __launch_bounds__(256) void trtri_diag_kernel(int n, const T *L, int ldl, T *Linv, int ldi) {
-    __shared__ T Ls[NB][NB + 1];
-    __shared__ T X[NB][NB + 1];
+__global__ __launch_bounds__(64) void trtri_diag_kernel(int n, const T *L, int ldl, T *Linv, int ldi) {
     const int o = blockIdx.x * NB;
     const int kb = (n - o < NB) ? n - o : NB;
-    const int tid = threadIdx.x;
-    for (int e = tid; e < kb * kb; e += 256) {
-        int r = e % kb, c = e / kb;
-        Ls[r][c] = (r >= c) ? L[(o + r) + (long) (o + c) * ldl] : (T) 0;
+    const int r = threadIdx.x;
+    T a[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        T v = (r == c) ? (T) 1 : (T) 0;
+        if (r < kb && c < kb && r >= c) v = L[(o + r) + (long) (o + c) * ldl];
+        a[c] = v;
     }
-    __syncthreads();
-    if (tid < kb) {
-        int c = tid;
-        X[c][c] = (T) 1 / Ls[c][c];
-        for (int r = c + 1; r < kb; ++r) {
-            T sum = 0;
-            for (int l = c; l < r; ++l) sum += Ls[r][l] * X[l][c];
-            X[r][c] = -sum / Ls[r][r];
-        }
-    }
-    __syncthreads();
-    for (int e = tid; e < kb * kb; e += 256) {
-        int r = e % kb, c = e / kb;
-        Linv[(o + r) + (long) (o + c) * ldi] = (r >= c) ? X[r][c] : (T) 0;
+    T x[NB];
+    leaf_inverse<T>(a, r, x);
+    if (r < kb) {
+#pragma unroll
+        for (int k = 0; k < NB; ++k)
+            if (k < kb) Linv[(o + k) + (long) (o + r) * ldi] = (k >= r) ? x[k] : (T) 0;
     }
 }
 
@@ -254,23 +466,24 @@ template <typename T>
 void trtri(hipStream_t s, int n, const T *L, int ldl, T *Linv, int ldi, T *work) {
     fill_matrix_kernel<T><<<512, 256, 0, s>>>(n, Linv, ldi, (T) 0);
     int nblk = (n + NB - 1) / NB;
-    trtri_diag_kernel<T><<<nblk, 256, 0, s>>>(n, L, ldl, Linv, ldi);
+    trtri_diag_kernel<T><<<nblk, NB, 0, s>>>(n, L, ldl, Linv, ldi);
     for (int sz = NB; sz < n; sz *= 2) {
         int npair = n / (2 * sz);
         long strideL = 2L * sz * (ldl + 1), strideI = 2L * sz * (ldi + 1), strideW = (long) sz * sz;
         if (npair > 0) {
+            // W = L21 Linv11 (Linv11 lower), then Linv21 = -Linv22 W (Linv22 lower)
             gemm<T>(s, false, false, false, sz, sz, sz, (T) 1, L + sz, ldl, strideL, Linv, ldi, strideI, (T) 0,
-                    work, sz, strideW, npair);
+                    work, sz, strideW, npair, 1);
             gemm<T>(s, false, false, false, sz, sz, sz, (T) -1, Linv + (long) sz * (ldi + 1), ldi, strideI, work,
-                    sz, strideW, (T) 0, Linv + sz, ldi, strideI, npair);
+                    sz, strideW, (T) 0, Linv + sz, ldi, strideI, npair, 2);
         }
         int o = npair * 2 * sz, rem = n - o;
         if (rem > sz) {
             int sc = rem - sz;
             gemm<T>(s, false, false, false, sc, sz, sz, (T) 1, L + (o + sz) + (long) o * ldl, ldl, 0,
-                    Linv + o + (long) o * ldi, ldi, 0, (T) 0, work, sc, 0, 1);
+                    Linv + o + (long) o * ldi, ldi, 0, (T) 0, work, sc, 0, 1, 1);
             gemm<T>(s, false, false, false, sc, sz, sc, (T) -1, Linv + (long) (o + sz) * (ldi + 1), ldi, 0, work,
-                    sc, 0, (T) 0, Linv + (o + sz) + (long) o * ldi, ldi, 0, 1);
+                    sc, 0, (T) 0, Linv + (o + sz) + (long) o * ldi, ldi, 0, 1, 2);
         }
     }
 }

@@ -26,15 +26,26 @@ constexpr int NB = 64;     // Cholesky / triangular-inverse block
 
 // C = alpha * op(A) * op(B) + beta * C over `batch` problems (column-major).
 // lower != 0 restricts the output to i >= j (tiles above the diagonal skipped).
+// tri = 1 when op(B) is lower triangular, 2 when op(A) is: the k ranges that
+// multiply the zero triangle are skipped (the stored zeros must be present).
 template <typename T>
 void gemm(hipStream_t s, bool transA, bool transB, bool lower, int M, int N, int K, T alpha,
           const T *A, int lda, long strideA, const T *B, int ldb, long strideB, T beta, T *C,
-          int ldc, long strideC, int batch = 1);
+          int ldc, long strideC, int batch = 1, int tri = 0);
+
+// A second stream and a ring of events for look-ahead factorisations.
+constexpr int LA_EVENTS = 16;
+struct LookAhead {
+    hipStream_t s2;
+    hipEvent_t ev[LA_EVENTS];
+};
 
 // In-place lower Cholesky of the n x n matrix A (lda), blocked by NB.
 // work: NB*NB scratch; info: device int, 0 on success else failing column + 1.
+// With la, the bulk of each trailing update runs on la->s2 while the next block
+// column is factored on s; s has waited for s2 when the call returns.
 template <typename T>
-void potrf(hipStream_t s, int n, T *A, int lda, T *work, int *info);
+void potrf(hipStream_t s, int n, T *A, int lda, T *work, int *info, const LookAhead *la = nullptr);
 
 // Linv = L^-1 for the lower-triangular n x n factor L (upper of Linv zeroed).
 // work: n * n scratch.

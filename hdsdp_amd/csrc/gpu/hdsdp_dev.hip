@@ -23,6 +23,7 @@ using hdk::DevCoeffs;
 struct hdsdp_dev {
     int m, n, precision;
     hipStream_t st;
+    hdk::LookAhead la;  // second stream + events for look-ahead factorisations
     std::vector<void *> allocs;
     DevCoeffs c;
     int cType;
@@ -199,7 +200,7 @@ hdsdp_retcode lanczos(hdsdp_dev *d, int f, double *maxStep) {
     HIP_OK(hipMemcpyAsync(d->lw, d->hvec_n, n * sizeof(double), hipMemcpyHostToDevice, d->st));
     hdk::lz_start(d->st, n, d->lw, d->lv, d->V);
     const int check = std::min(maxDim / 5, 3);
-    double U[31 * 31], evals[2], Y[2 * 31];
+    double Td[32], Te[32], evals[2], Y[2 * 32];
     for (int k = 0; k < maxDim; ++k) {
         lz_mvec(d, f, d->lv, d->lw);
         hdk::lz_step(d->st, n, k, d->lw, d->V, d->Hd, nH, d->lv, d->sc_d);
@@ -209,8 +210,11 @@ hdsdp_retcode lanczos(hdsdp_dev *d, int f, double *maxStep) {
         if (normPres > 0.0) H[(k + 1) + k * nH] = H[k + (k + 1) * nH] = normPres;
         if ((k + 1) % check == 0 || k > maxDim - 1 || normPres == 0.0) {
             const int kp1 = k + 1;
-            for (int i = 0; i < kp1; ++i) memcpy(U + kp1 * i, &H[nH * i], sizeof(double) * kp1);
-            if (kp1 < 2 || hdsdp_eig_small_top(kp1, U, 2, evals, Y) != 0) return HDSDP_RETCODE_FAILED;
+            for (int i = 0; i < kp1; ++i) {
+                Td[i] = H[i + nH * i];
+                if (i + 1 < kp1) Te[i] = H[(i + 1) + nH * i];
+            }
+            if (kp1 < 2 || hdsdp_eig_tridiag_top(kp1, Td, Te, 2, evals, Y) != 0) return HDSDP_RETCODE_FAILED;
             double resi = fabs(H[kp1 + k * nH] * Y[kp1 + k]);
             if (resi < 1e-04 || k >= maxDim - 1) {
                 double e1 = evals[1], e2 = evals[0];
@@ -261,7 +265,7 @@ hdsdp_retcode build_precond(hdsdp_dev *d) {
         }
         prof_begin(d, HDSDP_PROF_MFACTOR);
         hdk::convert_lower_to_float(d->st, m, d->M, m, d->Mf, m);
-        hdk::potrf<float>(d->st, m, d->Mf, m, d->workf, d->info_d);
+        hdk::potrf<float>(d->st, m, d->Mf, m, d->workf, d->info_d, &d->la);
         HIP_OK(hipMemcpyAsync(d->hinfo, d->info_d, sizeof(int), hipMemcpyDeviceToHost, d->st));
         DCALL(sync(d));
         if (*d->hinfo != 0) return HDSDP_RETCODE_FAILED;
@@ -277,7 +281,7 @@ hdsdp_retcode build_precond(hdsdp_dev *d) {
     }
     prof_begin(d, HDSDP_PROF_MFACTOR);
     HIP_OK(hipMemcpyAsync(d->Mchol, d->M, sizeof(double) * (size_t) m * m, hipMemcpyDeviceToDevice, d->st));
-    hdk::potrf<double>(d->st, m, d->Mchol, m, d->MLiT, d->info_d);
+    hdk::potrf<double>(d->st, m, d->Mchol, m, d->MLiT, d->info_d, &d->la);
     HIP_OK(hipMemcpyAsync(d->hinfo, d->info_d, sizeof(int), hipMemcpyDeviceToHost, d->st));
     DCALL(sync(d));
     if (*d->hinfo != 0) return HDSDP_RETCODE_FAILED;
@@ -406,6 +410,8 @@ hdsdp_retcode hdsdp_dev_create(hdsdp_dev **pdev, int nRow, int nCol, int precisi
     d->n = nCol;
     d->precision = precision;
     HIP_OK(hipStreamCreateWithFlags(&d->st, hipStreamNonBlocking));
+    HIP_OK(hipStreamCreateWithFlags(&d->la.s2, hipStreamNonBlocking));
+    for (int k = 0; k < hdk::LA_EVENTS; ++k) HIP_OK(hipEventCreateWithFlags(&d->la.ev[k], hipEventDisableTiming));
     const size_t n = nCol, m = nRow, nn = n * n;
     for (int k = 0; k < 3; ++k) DCALL(dalloc(d, &d->S[k], nn));
     for (int k = 0; k < 2; ++k) {
@@ -717,6 +723,12 @@ void hdsdp_dev_destroy(hdsdp_dev **pdev) {
         if (d->evA[k]) hipEventDestroy(d->evA[k]);
         if (d->evB[k]) hipEventDestroy(d->evB[k]);
     }
+    for (int k = 0; k < hdk::LA_EVENTS; ++k)
+        if (d->la.ev[k]) hipEventDestroy(d->la.ev[k]);
+    if (d->la.s2) {
+        hipStreamSynchronize(d->la.s2);
+        hipStreamDestroy(d->la.s2);
+    }
     if (d->st) hipStreamDestroy(d->st);
     delete d;
     *pdev = nullptr;
@@ -767,7 +779,7 @@ hdsdp_retcode hdsdp_dev_factorize(hdsdp_dev *d, int which, int *isPsd) {
     const int n = d->n, f = fidx(which);
     prof_begin(d, HDSDP_PROF_SFACTOR);
     HIP_OK(hipMemcpyAsync(d->L[f], d->S[f], sizeof(double) * (size_t) n * n, hipMemcpyDeviceToDevice, d->st));
-    hdk::potrf<double>(d->st, n, d->L[f], n, d->T1, d->info_d);
+    hdk::potrf<double>(d->st, n, d->L[f], n, d->T1, d->info_d, &d->la);
     HIP_OK(hipMemcpyAsync(d->hinfo, d->info_d, sizeof(int), hipMemcpyDeviceToHost, d->st));
     DCALL(prof_end(d, HDSDP_PROF_SFACTOR));
     DCALL(sync(d));

@@ -7,10 +7,12 @@ extern "C" {
 
 #define HDSDP_EIG_SMALL_MAX 64
 
-/* nTop largest eigenvalues of the symmetric n x n column-major A (n <= 64),
- * ascending in evals[0..nTop-1], eigenvectors as columns of evecs (n x nTop,
- * may be NULL).  Returns 0 on success, -1 on invalid sizes. */
-int hdsdp_eig_small_top( int n, const double *A, int nTop, double *evals, double *evecs );
+/* nTop largest eigenvalues of the symmetric tridiagonal n x n matrix with
+ * diagonal d[0..n-1] and off-diagonal e[0..n-2] (n <= 64), ascending in
+ * evals[0..nTop-1]; unit eigenvectors as columns of evecs (n x nTop, may be
+ * NULL), largest-magnitude component positive.  Returns 0 on success, -1 on
+ * invalid sizes. */
+int hdsdp_eig_tridiag_top( int n, const double *d, const double *e, int nTop, double *evals, double *evecs );
 
 #ifdef __cplusplus
 }

@@ -511,8 +511,13 @@ hdsdp_retcode hdsdp_dev_ratio_test( hdsdp_dev *d, int which, double *maxStep ) {
         }
         if ( (k + 1) % check == 0 || k > lz->maxDim - 1 || normPres == 0.0 ) {
             int kp1 = k + 1;
-            for ( int i = 0; i < kp1; ++i ) memcpy(lz->U + (size_t) kp1 * i, &HH(0, i), sizeof(double) * kp1);
-            if ( kp1 < 2 || hdsdp_eig_small_top(kp1, lz->U, 2, lz->d, lz->Y) != 0 ) return HDSDP_RETCODE_FAILED;
+            /* projected matrix: the leading (k+1) x (k+1) Lanczos tridiagonal */
+            for ( int i = 0; i < kp1; ++i ) {
+                lz->U[i] = HH(i, i);
+                if ( i + 1 < kp1 ) lz->U[kp1 + i] = HH(i + 1, i);
+            }
+            if ( kp1 < 2 || hdsdp_eig_tridiag_top(kp1, lz->U, lz->U + kp1, 2, lz->d, lz->Y) != 0 )
+                return HDSDP_RETCODE_FAILED;
             double resi = fabs(HH(kp1, k) * lz->Y[kp1 + k]);
             if ( trace ) printf("Lanczos outer resi value: %f \n", resi);
             if ( resi < 1e-04 || k >= lz->maxDim - 1 ) {

@@ -140,25 +140,40 @@ def test_primal_recovery(oracle, name):
     assert abs(tx - np.trace(Xref)) <= 1e-10 * abs(np.trace(Xref))
 
 
-def test_eig_small_matches_numpy(oracle):
+def test_tridiagonal_top_eigenpairs_match_numpy(oracle):
+    """The projected-matrix eigen-solve of the Lanczos routine (two largest
+    eigenpairs of a symmetric tridiagonal) against numpy.linalg.eigh, including
+    a Lanczos-like matrix with a nearly repeated top eigenvalue."""
     import ctypes as C
 
-    fn = oracle.hdsdp_eig_small_top
+    P = C.POINTER(C.c_double)
+    fn = oracle.hdsdp_eig_tridiag_top
     fn.restype = C.c_int
-    fn.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    fn.argtypes = [C.c_int, P, P, C.c_int, P, P]
     rng = np.random.default_rng(5)
-    for n in (2, 3, 7, 19, 31):
-        A = rng.standard_normal((n, n))
-        A = A + A.T
+    cases = [(rng.standard_normal(n), rng.standard_normal(n - 1)) for n in (2, 3, 7, 19, 31, 64)]
+    d = np.array([1.0, 2.0, 3.0, 3.0 + 1e-9, 0.5])
+    cases.append((d, np.array([1e-3, 1e-3, 1e-12, 1e-3])))
+    cases.append((np.zeros(6), np.ones(5)))
+    for d, e in cases:
+        n = len(d)
+        T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
         ev = np.zeros(2)
         V = np.zeros(2 * n)
-        assert fn(n, np.asfortranarray(A).ctypes.data_as(C.POINTER(C.c_double)), 2,
-                  ev.ctypes.data_as(C.POINTER(C.c_double)), V.ctypes.data_as(C.POINTER(C.c_double))) == 0
-        w, U = np.linalg.eigh(A)
-        assert np.allclose(ev, w[-2:], rtol=1e-12, atol=1e-12)
+        assert fn(n, np.ascontiguousarray(d).ctypes.data_as(P), np.ascontiguousarray(e).ctypes.data_as(P), 2,
+                  ev.ctypes.data_as(P), V.ctypes.data_as(P)) == 0
+        w, U = np.linalg.eigh(T)
+        scale = max(1.0, np.abs(w).max())
+        assert np.allclose(ev, w[-2:], rtol=0, atol=4e-15 * scale * n)
+        V = V.reshape(2, n)
         for k in range(2):
-            v = V[k * n:(k + 1) * n]
-            assert abs(abs(v @ U[:, n - 2 + k]) - 1.0) < 1e-10
+            # residual of the computed pair, and orthonormality of the two vectors
+            assert np.linalg.norm(T @ V[k] - ev[k] * V[k]) <= 1e-12 * scale
+            assert abs(np.linalg.norm(V[k]) - 1.0) < 1e-12
+        assert abs(V[0] @ V[1]) < 1e-10
+        # wrong-row check: the Sturm count planted at a shifted index misses the top pair
+        if n > 3 and w[-3] < w[-2] - 1e-6:
+            assert abs(ev[0] - w[-3]) > 1e-7
 
 
 # ---- known optimal values ---------------------------------------------------------------
