@@ -632,6 +632,22 @@ void convert_lower_to_float(hipStream_t s, int n, const double *A, int lda, floa
     convert_lower_kernel<<<2048, 256, 0, s>>>(n, A, lda, B, ldb);
 }
 
+__global__ void convert_lower_scaled_kernel(int n, const double *A, int lda, const double *sc, double shift, float *B,
+                                            int ldb) {
+    long total = (long) n * n;
+    for (long e = blockIdx.x * (long) blockDim.x + threadIdx.x; e < total; e += (long) gridDim.x * blockDim.x) {
+        long i = e % n, j = e / n;
+        float v = 0.0f;
+        if (i >= j) v = (float) (A[i + j * lda] * sc[i] * sc[j] + ((i == j) ? shift : 0.0));
+        B[i + j * ldb] = v;
+    }
+}
+
+void convert_lower_scaled(hipStream_t s, int n, const double *A, int lda, const double *sc, double shift, float *B,
+                          int ldb) {
+    convert_lower_scaled_kernel<<<2048, 256, 0, s>>>(n, A, lda, sc, shift, B, ldb);
+}
+
 __global__ void add_diag_kernel(int n, double *A, int lda, const double *d, double scalar) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) A[i + (long) i * lda] += (d ? d[i] : 0.0) + scalar;

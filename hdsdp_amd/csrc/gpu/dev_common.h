@@ -71,6 +71,9 @@ void fill(hipStream_t s, double *x, long n, double v);
 void fill_f(hipStream_t s, float *x, long n, float v);
 void copy_lower(hipStream_t s, int n, const double *A, int lda, double *B, int ldb);
 void convert_lower_to_float(hipStream_t s, int n, const double *A, int lda, float *B, int ldb);
+// B = lower(diag(sc) A diag(sc) + shift I) in FP32, zeros above the diagonal
+void convert_lower_scaled(hipStream_t s, int n, const double *A, int lda, const double *sc, double shift, float *B,
+                          int ldb);
 void add_diag(hipStream_t s, int n, double *A, int lda, const double *d, double scalar);
 // out[0] = 2 * sum log A_jj
 void logdet_diag(hipStream_t s, int n, const double *A, int lda, double *out);

@@ -20,6 +20,9 @@
 
 using hdk::DevCoeffs;
 
+// preconditioner of the Schur CG
+enum { PREC_JACOBI = 0, PREC_CHOL64 = 1, PREC_CHOL32 = 2 };
+
 struct hdsdp_dev {
     int m, n, precision;
     hipStream_t st;
@@ -59,6 +62,9 @@ struct hdsdp_dev {
     double *M, *Mchol, *MLi, *MLiT;
     float *Mf, *MLif, *MLiTf, *cf1, *cf2, *workf;
     double *jac, *cr, *crn, *cp, *cz, *cMp, *cx, *crhs, *ctmp;
+    double *precS;     // Jacobi scaling of the FP32 preconditioner
+    double precShift;  // diagonal shift the FP32 factorisation needed
+    int precKind;
     int useJacobi, paramUseJacobi, maxIter, nRestart;
     double absTol, relTol;
     double stats[6];
@@ -247,39 +253,14 @@ hdsdp_retcode lanczos(hdsdp_dev *d, int f, double *maxStep) {
     return check_launch();
 }
 
-hdsdp_retcode build_precond(hdsdp_dev *d) {
+// FP64 Cholesky preconditioner: M = L L^T, kept as L^-1 and L^-T.
+hdsdp_retcode build_chol64(hdsdp_dev *d) {
     const int m = d->m;
-    if (d->useJacobi) {
-        hdk::get_diag(d->st, m, d->M, m, d->jac);
-        return check_launch();
-    }
-    d->stats[4] += 1;
-    if (d->precision == HDSDP_PRECISION_MIXED) {
-        if (!d->Mf) {
-            DCALL(dalloc(d, &d->Mf, (size_t) m * m));
-            DCALL(dalloc(d, &d->MLif, (size_t) m * m));
-            DCALL(dalloc(d, &d->MLiTf, (size_t) m * m));
-            DCALL(dalloc(d, &d->cf1, (size_t) m));
-            DCALL(dalloc(d, &d->cf2, (size_t) m));
-            DCALL(dalloc(d, &d->workf, (size_t) m * m));
-        }
-        prof_begin(d, HDSDP_PROF_MFACTOR);
-        hdk::convert_lower_to_float(d->st, m, d->M, m, d->Mf, m);
-        hdk::potrf<float>(d->st, m, d->Mf, m, d->workf, d->info_d, &d->la);
-        HIP_OK(hipMemcpyAsync(d->hinfo, d->info_d, sizeof(int), hipMemcpyDeviceToHost, d->st));
-        DCALL(sync(d));
-        if (*d->hinfo != 0) return HDSDP_RETCODE_FAILED;
-        hdk::trtri<float>(d->st, m, d->Mf, m, d->MLif, m, d->workf);
-        hdk::transpose<float>(d->st, m, d->MLif, m, d->MLiTf, m);
-        DCALL(check_launch());
-        return prof_end(d, HDSDP_PROF_MFACTOR);
-    }
     if (!d->Mchol) {
         DCALL(dalloc(d, &d->Mchol, (size_t) m * m));
         DCALL(dalloc(d, &d->MLi, (size_t) m * m));
         DCALL(dalloc(d, &d->MLiT, (size_t) m * m));
     }
-    prof_begin(d, HDSDP_PROF_MFACTOR);
     HIP_OK(hipMemcpyAsync(d->Mchol, d->M, sizeof(double) * (size_t) m * m, hipMemcpyDeviceToDevice, d->st));
     hdk::potrf<double>(d->st, m, d->Mchol, m, d->MLiT, d->info_d, &d->la);
     HIP_OK(hipMemcpyAsync(d->hinfo, d->info_d, sizeof(int), hipMemcpyDeviceToHost, d->st));
@@ -287,22 +268,81 @@ hdsdp_retcode build_precond(hdsdp_dev *d) {
     if (*d->hinfo != 0) return HDSDP_RETCODE_FAILED;
     hdk::trtri<double>(d->st, m, d->Mchol, m, d->MLi, m, d->MLiT);
     hdk::transpose<double>(d->st, m, d->MLi, m, d->MLiT, m);
-    DCALL(check_launch());
+    d->precKind = PREC_CHOL64;
+    return check_launch();
+}
+
+// FP32 Cholesky preconditioner of the equilibrated matrix S M S (S = diag(M)^-1/2):
+// unit diagonal, so the FP32 factorisation sees the best-scaled form; when it still
+// breaks down (M too ill-conditioned for FP32) a growing diagonal shift is added.
+// The preconditioner only steers the FP64 CG, whose residuals decide convergence.
+hdsdp_retcode build_chol32(hdsdp_dev *d, int *ok) {
+    const int m = d->m;
+    if (!d->Mf) {
+        DCALL(dalloc(d, &d->Mf, (size_t) m * m));
+        DCALL(dalloc(d, &d->MLif, (size_t) m * m));
+        DCALL(dalloc(d, &d->MLiTf, (size_t) m * m));
+        DCALL(dalloc(d, &d->cf1, (size_t) m));
+        DCALL(dalloc(d, &d->cf2, (size_t) m));
+        DCALL(dalloc(d, &d->workf, (size_t) m * m));
+        DCALL(dalloc(d, &d->precS, (size_t) m));
+    }
+    *ok = 0;
+    hdk::jacobi_scale(d->st, m, d->M, m, d->precS);
+    // M only grows more ill-conditioned along the path: start from the shift the
+    // previous factorisation needed
+    const double shifts[] = {0.0, 1e-6, 1e-5, 1e-4, 1e-3, 1e-2};
+    for (double shift : shifts) {
+        if (shift < d->precShift) continue;
+        hdk::convert_lower_scaled(d->st, m, d->M, m, d->precS, shift, d->Mf, m);
+        hdk::potrf<float>(d->st, m, d->Mf, m, d->workf, d->info_d, &d->la);
+        HIP_OK(hipMemcpyAsync(d->hinfo, d->info_d, sizeof(int), hipMemcpyDeviceToHost, d->st));
+        DCALL(sync(d));
+        if (*d->hinfo == 0) {
+            d->precShift = shift;
+            *ok = 1;
+            break;
+        }
+    }
+    if (!*ok) return HDSDP_RETCODE_OK;
+    hdk::trtri<float>(d->st, m, d->Mf, m, d->MLif, m, d->workf);
+    hdk::transpose<float>(d->st, m, d->MLif, m, d->MLiTf, m);
+    d->precKind = PREC_CHOL32;
+    return check_launch();
+}
+
+hdsdp_retcode build_precond(hdsdp_dev *d) {
+    const int m = d->m;
+    if (d->useJacobi) {
+        hdk::get_diag(d->st, m, d->M, m, d->jac);
+        d->precKind = PREC_JACOBI;
+        return check_launch();
+    }
+    d->stats[4] += 1;
+    prof_begin(d, HDSDP_PROF_MFACTOR);
+    if (d->precision == HDSDP_PRECISION_MIXED) {
+        int ok = 0;
+        DCALL(build_chol32(d, &ok));
+        if (!ok) DCALL(build_chol64(d));
+    } else {
+        DCALL(build_chol64(d));
+    }
     return prof_end(d, HDSDP_PROF_MFACTOR);
 }
 
 // z = P^-1 r (and optionally p = z)
 void apply_precond(hdsdp_dev *d, const double *r, double *z, double *p) {
     const int m = d->m;
-    if (d->useJacobi) {
+    if (d->precKind == PREC_JACOBI) {
         hdk::cg_jacobi(d->st, m, r, d->jac, z, p);
         return;
     }
-    if (d->precision == HDSDP_PRECISION_MIXED) {
-        hdk::d2f(d->st, m, r, d->cf1);
+    if (d->precKind == PREC_CHOL32) {
+        // z = S (L^-T L^-1) S r with the FP32 factor of S M S
+        hdk::d2f_scaled(d->st, m, r, d->precS, d->cf1);
         hdk::coldot<float>(d->st, m, m, 2, 1.0f, d->MLiTf, m, d->cf1, 0.0f, d->cf2);
         hdk::coldot<float>(d->st, m, m, 1, 1.0f, d->MLif, m, d->cf2, 0.0f, d->cf1);
-        hdk::f2d(d->st, m, d->cf1, z);
+        hdk::f2d_scaled(d->st, m, d->cf1, d->precS, z);
     } else {
         hdk::coldot<double>(d->st, m, m, 2, 1.0, d->MLiT, m, r, 0.0, d->ctmp);
         hdk::coldot<double>(d->st, m, m, 1, 1.0, d->MLi, m, d->ctmp, 0.0, z);
@@ -328,7 +368,8 @@ int cg_solve(hdsdp_dev *d) {
     if (rnorm < tol) return 1;
     apply_precond(d, d->cr, d->cz, d->cp);
     symv_M(d, d->cp, d->cMp);
-    int iter = 0, status = 0;
+    int iter = 0, status = 0, refine = 0;
+    double prevTrue = 0.0;
     for (iter = 0; iter < d->maxIter; ++iter) {
         hdk::cg_step1(d->st, m, d->cz, d->cr, d->cp, d->cMp, d->cx, d->crn, d->sc_d);
         d->stats[2] += 1;
@@ -350,14 +391,40 @@ int cg_solve(hdsdp_dev *d) {
             status = 2;
             break;
         }
-        if (rnorm < tol) break;
+        if (rnorm < tol) {
+            // With the FP32 preconditioner the recursively updated residual drifts
+            // from b - M x: confirm convergence on the true FP64 residual and, if it
+            // is not met, restart from x (iterative refinement).
+            // Refinement stops at tol or once the true residual stops halving (the
+            // rounding level of an FP64 product with M).
+            if (d->precision != HDSDP_PRECISION_MIXED || d->useJacobi || refine >= 8) break;
+            symv_M(d, d->cx, d->ctmp);
+            hdk::cg_true_residual(d->st, m, d->crhs, d->ctmp, d->cr, d->sc_d);
+            if (read_scalars(d) != HDSDP_RETCODE_OK) return -1;
+            const double rtrue = d->hsc[hdk::CG_TRUE];
+            if (rtrue < tol || (refine > 0 && rtrue > 0.5 * prevTrue)) break;
+            prevTrue = rtrue;
+            ++refine;
+            apply_precond(d, d->cr, d->cz, d->cp);
+            symv_M(d, d->cp, d->cMp);
+        }
     }
     d->stats[5] = iter;
+    static const int trace = getenv("HDSDP_TRACE_CG") != nullptr;
+    if (trace)
+        fprintf(stderr, "[hdsdp_amd] cg m=%d precond=%s iters=%d refine=%d rnorm=%.3e bnorm=%.3e tol=%.3e status=%d\n",
+                m, d->useJacobi ? "jacobi" : (d->precision == HDSDP_PRECISION_MIXED ? "chol-fp32" : "chol-fp64"), iter,
+                refine, rnorm, bnorm, tol, status);
     if (iter >= d->maxIter || status == 2) {
         if (d->useJacobi) {
             d->useJacobi = 0;
             d->paramUseJacobi = 0;
             if (build_precond(d) != HDSDP_RETCODE_OK) return -1;
+            return cg_solve(d);
+        }
+        if (d->precKind == PREC_CHOL32) {
+            // the FP32 preconditioner was not good enough for this system: FP64 factor
+            if (build_chol64(d) != HDSDP_RETCODE_OK) return -1;
             return cg_solve(d);
         }
         return -1;

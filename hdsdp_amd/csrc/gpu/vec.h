@@ -17,6 +17,7 @@ enum {
     LZ_R2 = 7,
     SC_LOGDET = 8,
     SC_AUX = 9,
+    CG_TRUE = 11,
     SC_SLOTS = 16
 };
 
@@ -26,6 +27,8 @@ void cg_step1(hipStream_t s, int m, const double *z, const double *r, const doub
               double *rn, double *sc);
 void cg_step2(hipStream_t s, int m, const double *rn, const double *z, double *p, double *r, double *sc);
 void cg_residual(hipStream_t s, int m, const double *rhs, const double *Mx, double *r);
+// r = rhs - Mx and sc[CG_TRUE] = ||r||
+void cg_true_residual(hipStream_t s, int m, const double *rhs, const double *Mx, double *r, double *sc);
 
 void lz_step(hipStream_t s, int n, int k, double *w, double *V, double *H, int ldh, double *v, double *sc);
 void lz_start(hipStream_t s, int n, const double *x, double *v, double *V);
@@ -36,6 +39,11 @@ void axpy(hipStream_t s, long n, double a, const double *x, double *y);
 void regularize(hipStream_t s, int m, double *M, long ldm, double r);
 void get_diag(hipStream_t s, int m, const double *M, long ldm, double *d);
 void f2d(hipStream_t s, long n, const float *x, double *y);
+// y = x * sc (elementwise, with the precision conversion)
+void f2d_scaled(hipStream_t s, long n, const float *x, const double *sc, double *y);
+void d2f_scaled(hipStream_t s, long n, const double *x, const double *sc, float *y);
+// sc[i] = 1 / sqrt(M_ii) (1 where M_ii <= 0)
+void jacobi_scale(hipStream_t s, int m, const double *M, long ldm, double *sc);
 void d2f(hipStream_t s, long n, const double *x, float *y);
 
 }  // namespace hdk

@@ -98,6 +98,23 @@ __global__ void cg_residual_kernel(int m, const double *rhs, const double *Mx, d
     for (int i = threadIdx.x + blockIdx.x * blockDim.x; i < m; i += blockDim.x * gridDim.x) r[i] = rhs[i] - Mx[i];
 }
 
+__global__ __launch_bounds__(1024) void cg_true_residual_kernel(int m, const double *rhs, const double *Mx, double *r,
+                                                                double *sc) {
+    __shared__ double red[17];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+        const double v = rhs[i] - Mx[i];
+        r[i] = v;
+        acc += v * v;
+    }
+    const double nrm = sqrt(block_sum(acc, red));
+    if (threadIdx.x == 0) sc[CG_TRUE] = nrm;
+}
+
+void cg_true_residual(hipStream_t s, int m, const double *rhs, const double *Mx, double *r, double *sc) {
+    cg_true_residual_kernel<<<1, 1024, 0, s>>>(m, rhs, Mx, r, sc);
+}
+
 void cg_start(hipStream_t s, int m, const double *rhs, double *r, double *x, double *sc) {
     cg_start_kernel<<<1, 1024, 0, s>>>(m, rhs, r, x, sc);
 }
@@ -260,6 +277,31 @@ __global__ void f2d_kernel(long n, const float *x, double *y) {
 __global__ void d2f_kernel(long n, const double *x, float *y) {
     for (long i = threadIdx.x + blockIdx.x * (long) blockDim.x; i < n; i += (long) blockDim.x * gridDim.x)
         y[i] = (float) x[i];
+}
+
+__global__ void f2d_scaled_kernel(long n, const float *x, const double *sc, double *y) {
+    for (long i = threadIdx.x + blockIdx.x * (long) blockDim.x; i < n; i += (long) blockDim.x * gridDim.x)
+        y[i] = (double) x[i] * sc[i];
+}
+__global__ void d2f_scaled_kernel(long n, const double *x, const double *sc, float *y) {
+    for (long i = threadIdx.x + blockIdx.x * (long) blockDim.x; i < n; i += (long) blockDim.x * gridDim.x)
+        y[i] = (float) (x[i] * sc[i]);
+}
+__global__ void jacobi_scale_kernel(int m, const double *M, long ldm, double *sc) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) {
+        const double d = M[i + (long) i * ldm];
+        sc[i] = (d > 0.0) ? 1.0 / sqrt(d) : 1.0;
+    }
+}
+void f2d_scaled(hipStream_t s, long n, const float *x, const double *sc, double *y) {
+    f2d_scaled_kernel<<<(n + 255) / 256 > 4096 ? 4096 : (n + 255) / 256, 256, 0, s>>>(n, x, sc, y);
+}
+void d2f_scaled(hipStream_t s, long n, const double *x, const double *sc, float *y) {
+    d2f_scaled_kernel<<<(n + 255) / 256 > 4096 ? 4096 : (n + 255) / 256, 256, 0, s>>>(n, x, sc, y);
+}
+void jacobi_scale(hipStream_t s, int m, const double *M, long ldm, double *sc) {
+    jacobi_scale_kernel<<<(m + 255) / 256, 256, 0, s>>>(m, M, ldm, sc);
 }
 
 void f2d(hipStream_t s, long n, const float *x, double *y) {
