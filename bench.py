@@ -34,7 +34,21 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 REF_RUNNER = os.path.join(ROOT, "oracle", "_ref", "hdsdp_ref_run")
-FIXTURE = os.path.join(ROOT, "tests", "golden", "c2_gap1e-6")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+# workload -> (description, instance constructor, reference fixture or None)
+WORKLOADS = {
+    "C1": ("MaxCut SDP, G(n=200, p=0.05) seed 1, weights +-1: n=200, m=200",
+           lambda I: I.maxcut(200, 0.05, 1, "pm1"), "c1_gap1e-6"),
+    "C2": ("Lovasz theta SDP, G(n=400, p=0.1) seed 2: n=400, m=8084",
+           lambda I: I.lovasz_theta(400, 0.1, 2), "c2_gap1e-6"),
+    "C3": ("dense random SDP, n=1000, m=2000, seed 3",
+           lambda I: I.dense_random(1000, 2000, 3), None),
+    "C4": ("rank-one SDP (factor form), n=2000, m=20000, r=20, seed 4",
+           lambda I: I.rank_one(2000, 20000, 20, 4), None),
+    "C5": ("MaxCut SDP, G(n=2000, p=0.01) seed 5, weights U(0,1): n=2000, m=2000",
+           lambda I: I.maxcut(2000, 0.01, 5, "unif"), "c5_gap1e-6"),
+}
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
@@ -45,15 +59,20 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="C2")
+    ap.add_argument("--maxiter", type=int, default=None, help="cap IPM iterations (timing samples)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     return ap.parse_args()
 
 
-def parity(res):
-    with open(FIXTURE + ".json") as f:
+def parity(res, fixture):
+    if fixture is None:
+        return None
+    path = os.path.join(GOLDEN, fixture)
+    with open(path + ".json") as f:
         fix = json.load(f)
-    y0 = np.load(FIXTURE + "_y.npy")
+    y0 = np.load(path + "_y.npy")
     return {
         "vs": "reference HDSDP FP64 (CPU) on the same instance",
         "status": [res.status, fix["status"]],
@@ -108,9 +127,10 @@ def main():
     from hdsdp_amd.solver import load_library, run_solver
 
     lib = load_library()
-    inst = I.lovasz_theta(400, 0.1, 2)
+    desc, make, fixture = WORKLOADS[args.workload]
+    inst = make(I)
     prec = 1 if args.precision == "mixed" else 0
-    opts = dict(relopt=1e-6, absopt=1e30, precision=prec)
+    opts = dict(relopt=1e-6, absopt=1e30, precision=prec, maxiter=args.maxiter)
     for _ in range(args.warmup):
         run_solver(lib, inst, **opts)
         print(f"[bench] warmup solve done", file=sys.stderr, flush=True)
@@ -130,14 +150,28 @@ def main():
             for k in results[0].profile}
 
     m, n = inst.m, inst.n
-    # pair kernel: lower-triangle Schur entries between the small constraints
-    n_small = m - 1  # all but tr X = 1 (n diagonal entries) take the pair path
+    # Schur pair kernel: 8 bytes per lower-triangle entry of M between the small
+    # (at most 4 entries) sparse constraints, plus S^-1 read once
+    if inst.factor is None:
+        col_nnz = np.diff(inst.beg)[1:]
+        n_small = int(np.count_nonzero(col_nnz <= 4))
+    else:
+        n_small = 0
     pair_bytes = 8.0 * n_small * (n_small + 1) / 2 + 8.0 * n * n
     pair_ms, pair_cnt = prof["pairs"]
     pair_avg_s = pair_ms / 1e3 / max(pair_cnt, 1)
     achieved = pair_bytes / pair_avg_s / 1e9 if pair_cnt else 0.0
     mf_ms, mf_cnt = prof["mfactor"]
     mf_flops = 2.0 * m ** 3 / 3.0  # Cholesky (m^3/3) + triangular inverse (m^3/3)
+    # algorithmic flops of one Schur build of the Gram formulations (dense: T_i = L^-1 A_i L^-T
+    # by triangular products and the symmetric Gram of the T_i; rank-one: W F and F^T W F)
+    sc_ms, sc_cnt = prof["schur"]
+    if args.workload == "C3":
+        schur_flops = 2.0 * m * n ** 3 + m * (m + 1.0) * n ** 2
+    elif args.workload == "C4":
+        schur_flops = 1.0 * n * n * m + m * (m + 1.0) * n
+    else:
+        schur_flops = None
 
     line = {
         "metric": "IPM s/iter + solve s to 1e-6 gap, mixed vs FP64 GPU; Schur TFLOP/s vs peak",
@@ -151,13 +185,16 @@ def main():
         "scaling": "strong",
         "vs_baseline": None,
         "dtype": "f64" if prec == 0 else "f64+f32",
-        "data": "synthetic (seeded Erdos-Renyi graph, numpy PCG64 seed 2)",
-        "config": {"workload": "Lovasz theta SDP, G(n=400, p=0.1) seed 2: n=400, m=8084, FP64 GPU, "
-                               "gap 1e-6", "n": n, "m": m, "precision": args.precision},
+        "data": "synthetic (seeded instance generator, numpy PCG64)",
+        "config": {"workload": f"{desc}, {args.precision} GPU, gap 1e-6", "name": args.workload, "n": n, "m": m,
+                   "precision": args.precision, "maxiter": args.maxiter},
         "iters_per_solve": results[0].iters,
         "solve_s": elapsed / args.steps,
         "solve_s_to_gap": float(np.mean([r.seconds_to_gap for r in results])),
-        "parity": parity(results[-1]),
+        "status": results[-1].status,
+        "dobj": results[-1].dobj,
+        "pobj": results[-1].pobj,
+        "parity": parity(results[-1], fixture if args.maxiter is None else None),
         "roofline": {
             "bound": "hbm",
             "kernel": "schur_pair_kernel",
@@ -177,9 +214,17 @@ def main():
             "avg_ms": mf_ms / max(mf_cnt, 1),
             "count": mf_cnt,
         },
+        "schur_build": None if schur_flops is None or sc_cnt == 0 else {
+            "bound": "mfma",
+            "algorithmic_flops": schur_flops,
+            "avg_ms": sc_ms / sc_cnt,
+            "achieved_tflops": schur_flops / (sc_ms / sc_cnt / 1e3) / 1e12,
+            "peak_tflops": FP64_PEAK_TFLOPS,
+            "count": sc_cnt,
+        },
         "phases_ms_per_iter": {k: v[0] / iters for k, v in prof.items()},
     }
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and inst.factor is None:
         line["cpu_baseline"] = cpu_baseline(inst, args.cpu_threads)
     print(json.dumps(line))
 

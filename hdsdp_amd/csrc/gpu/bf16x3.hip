@@ -1,0 +1,195 @@
+// bf16x3.hip -- FP32-equivalent products on the bf16 matrix cores of gfx950.
+//
+// C (M x N) = alpha * X^T Y + beta * C for column-major X (K x M) and Y (K x N):
+// both operands are contiguous along k, which is the natural layout of the Gram
+// products of the Schur matrix (M = T^T T over vec(T_i), K = G^T G over the
+// columns of G) and of the two-sided products L^-1 A L^-T written as X^T Y.
+//
+// Each FP32 operand x is split on its way into LDS as x = hi + lo with
+// hi = bf16(x), lo = bf16(x - hi); the product is formed from three bf16 MFMA
+// products hi*hi + hi*lo + lo*hi (the lo*lo term is below the FP32 rounding of
+// the sum) accumulated in FP32 by v_mfma_f32_16x16x32_bf16.  Every FLUSH k-slabs
+// the FP32 accumulators are added into FP64 ones, so long sums (K = n^2) do not
+// build up FP32 rounding.
+//
+// Tiles: 128 x 128 per workgroup of four waves (2 x 2), each wave 4 x 4 MFMA
+// 16 x 16 blocks; k slabs of 32 double-buffered in LDS (hi and lo planes, rows
+// padded to 40 bf16 so a lane's 8 consecutive k are one aligned 16-byte read).
+// MFMA operand layout (measured on gfx950): A(i, k) for i = lane % 16 and
+// k = 8 (lane / 16) + t in element t; B(k, j) likewise with j = lane % 16;
+// D(i, j) at lane j + 16 (i / 4), element i % 4.
+#include "dev_common.h"
+
+#include <hip/hip_bf16.h>
+
+namespace hdk {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int XT = 128;     // output tile
+constexpr int XK = 32;      // k slab = one MFMA k
+constexpr int XPAD = 8;     // row padding (bf16)
+constexpr int FLUSH = 32;   // slabs between FP64 accumulations (k = 1024)
+
+__device__ inline void split_store(float x, __bf16 &hi, __bf16 &lo) {
+    const __bf16 h = (__bf16) x;
+    hi = h;
+    lo = (__bf16) (x - (float) h);
+}
+
+// Load 16 consecutive k of row r (a column of X or Y) into registers.
+template <bool CHECK>
+__device__ inline void load16(const float *P, int ld, int nrows, int K, int row, int k0, float (&v)[16]) {
+    const float *p = P + k0 + (long) row * ld;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = (!CHECK || (row < nrows && k0 + u < K)) ? p[u] : 0.0f;
+}
+
+__device__ inline void store16(__bf16 *hi, __bf16 *lo, const float (&v)[16]) {
+    bf16x8 h0, h1, l0, l1;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        __bf16 a, b;
+        split_store(v[u], a, b);
+        h0[u] = a;
+        l0[u] = b;
+        split_store(v[u + 8], a, b);
+        h1[u] = a;
+        l1[u] = b;
+    }
+    *reinterpret_cast<bf16x8 *>(hi) = h0;
+    *reinterpret_cast<bf16x8 *>(hi + 8) = h1;
+    *reinterpret_cast<bf16x8 *>(lo) = l0;
+    *reinterpret_cast<bf16x8 *>(lo + 8) = l1;
+}
+
+template <typename Tout, bool LOWER>
+__global__ __launch_bounds__(256) void gemm_bf16x3_tn_kernel(int M, int N, int K, double alpha, const float *X, int ldx,
+                                                             long sX, const float *Y, int ldy, long sY, double beta,
+                                                             Tout *C, int ldc, long sC) {
+    const int i0 = blockIdx.x * XT, j0 = blockIdx.y * XT;
+    if (LOWER && i0 + XT - 1 < j0) return;
+    X += (long) blockIdx.z * sX;
+    Y += (long) blockIdx.z * sY;
+    C += (long) blockIdx.z * sC;
+    __shared__ __attribute__((aligned(16))) __bf16 Ah[2][XT][XK + XPAD];
+    __shared__ __attribute__((aligned(16))) __bf16 Al[2][XT][XK + XPAD];
+    __shared__ __attribute__((aligned(16))) __bf16 Bh[2][XT][XK + XPAD];
+    __shared__ __attribute__((aligned(16))) __bf16 Bl[2][XT][XK + XPAD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wi = (wave & 1) * 64, wj = (wave >> 1) * 64;
+    const int lr = lane & 15, lg = lane >> 4;
+    // loader mapping: thread -> row tid / 2 of the tile, k half (tid & 1) * 16
+    const int lrow = tid >> 1, lkh = (tid & 1) * 16;
+
+    f32x4 acc[4][4];
+    double acc64[4][4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc64[a][b][r] = 0.0;
+        }
+
+    float vx[16], vy[16];
+    auto fetch = [&](int k0) {
+        const bool interior = (i0 + XT <= M) && (j0 + XT <= N) && (k0 + XK <= K);
+        if (interior) {
+            load16<false>(X, ldx, M, K, i0 + lrow, k0 + lkh, vx);
+            load16<false>(Y, ldy, N, K, j0 + lrow, k0 + lkh, vy);
+        } else {
+            load16<true>(X, ldx, M, K, i0 + lrow, k0 + lkh, vx);
+            load16<true>(Y, ldy, N, K, j0 + lrow, k0 + lkh, vy);
+        }
+    };
+    auto stash = [&](int buf) {
+        store16(&Ah[buf][lrow][lkh], &Al[buf][lrow][lkh], vx);
+        store16(&Bh[buf][lrow][lkh], &Bl[buf][lrow][lkh], vy);
+    };
+
+    fetch(0);
+    stash(0);
+    __syncthreads();
+    const int nslab = (K + XK - 1) / XK;
+    for (int sl = 0; sl < nslab; ++sl) {
+        const int cur = sl & 1;
+        const bool more = sl + 1 < nslab;
+        if (more) fetch((sl + 1) * XK);
+        bf16x8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            ah[f] = *reinterpret_cast<const bf16x8 *>(&Ah[cur][wi + 16 * f + lr][8 * lg]);
+            al[f] = *reinterpret_cast<const bf16x8 *>(&Al[cur][wi + 16 * f + lr][8 * lg]);
+            bh[f] = *reinterpret_cast<const bf16x8 *>(&Bh[cur][wj + 16 * f + lr][8 * lg]);
+            bl[f] = *reinterpret_cast<const bf16x8 *>(&Bl[cur][wj + 16 * f + lr][8 * lg]);
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+            }
+        if ((sl + 1) % FLUSH == 0 || !more) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) acc64[a][b][r] += (double) acc[a][b][r];
+                    acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+                }
+        }
+        if (more) stash(cur ^ 1);
+        __syncthreads();
+    }
+    // epilogue: loads of C (beta != 0) issued before any store
+    Tout cin[4][4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = i0 + wi + 16 * a + 4 * lg + r, j = j0 + wj + 16 * b + lr;
+                const bool ok = i < M && j < N && (!LOWER || i >= j);
+                cin[a][b][r] = (ok && beta != 0.0) ? C[i + (long) j * ldc] : (Tout) 0;
+            }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = i0 + wi + 16 * a + 4 * lg + r, j = j0 + wj + 16 * b + lr;
+                if (i < M && j < N && (!LOWER || i >= j))
+                    C[i + (long) j * ldc] = (Tout) (alpha * acc64[a][b][r] + beta * (double) cin[a][b][r]);
+            }
+}
+
+}  // namespace
+
+template <typename Tout>
+void gemm_bf16x3_tn(hipStream_t s, bool lower, int M, int N, int K, double alpha, const float *X, int ldx, long sX,
+                    const float *Y, int ldy, long sY, double beta, Tout *C, int ldc, long sC, int batch) {
+    if (M <= 0 || N <= 0 || batch <= 0) return;
+    dim3 grid((M + XT - 1) / XT, (N + XT - 1) / XT, batch);
+    if (lower)
+        gemm_bf16x3_tn_kernel<Tout, true><<<grid, 256, 0, s>>>(M, N, K, alpha, X, ldx, sX, Y, ldy, sY, beta, C, ldc, sC);
+    else
+        gemm_bf16x3_tn_kernel<Tout, false><<<grid, 256, 0, s>>>(M, N, K, alpha, X, ldx, sX, Y, ldy, sY, beta, C, ldc,
+                                                                sC);
+}
+
+template void gemm_bf16x3_tn<float>(hipStream_t, bool, int, int, int, double, const float *, int, long, const float *,
+                                    int, long, double, float *, int, long, int);
+template void gemm_bf16x3_tn<double>(hipStream_t, bool, int, int, int, double, const float *, int, long,
+                                     const float *, int, long, double, double *, int, long, int);
+
+}  // namespace hdk

@@ -33,6 +33,13 @@ void gemm(hipStream_t s, bool transA, bool transB, bool lower, int M, int N, int
           const T *A, int lda, long strideA, const T *B, int ldb, long strideB, T beta, T *C,
           int ldc, long strideC, int batch = 1, int tri = 0);
 
+// C = alpha * X^T Y + beta * C in FP32-equivalent precision on the bf16 matrix cores
+// (three-term bf16 split products, FP32 accumulation flushed into FP64 every 1024 k).
+// X: K x M (ldx), Y: K x N (ldy), column-major FP32; lower restricts C to i >= j.
+template <typename Tout>
+void gemm_bf16x3_tn(hipStream_t s, bool lower, int M, int N, int K, double alpha, const float *X, int ldx, long sX,
+                    const float *Y, int ldy, long sY, double beta, Tout *C, int ldc, long sC, int batch = 1);
+
 // A second stream and a ring of events for look-ahead factorisations.
 constexpr int LA_EVENTS = 16;
 struct LookAhead {

@@ -42,6 +42,18 @@ struct hdsdp_dev {
     double *Gpart;           // gramParts x nDn^2
     int gramBatch, gramParts;
     long gramChunk;
+    // mixed precision Gram builds (bf16 split products): FP32 copies and products
+    float *LiTf;             // n x n, L^-T of the dual factor
+    float *Ff, *Gf;          // n x nR1: rank-one factors and G = L^-1 F
+    float *Tf, *Dbuff, *Vf;  // dense: T_p (nDn x n^2), batch buffers
+    // adaptive precision: the Gram blocks of M come from bf16 split products while
+    // schurMixed; when the system they form cannot be factored or solved, they are
+    // rebuilt in FP64 from the S^-1 and L^-1 of their build (Wm, Lim), the diagonal
+    // terms added to M since its build (diag(M) - mDiag0) are carried over, and later
+    // builds stay FP64
+    int schurMixed, mGramPending;
+    double *Wm, *Lim, *mDiag0, *mDiagCur;
+    double *xCoef;  // m + 1 coefficients of the matrix-free Schur product
 
     double *S[3], *L[2], *Li[2], *LiT[2];
     long facVer[2], liVer[2];
@@ -313,7 +325,9 @@ hdsdp_retcode build_chol32(hdsdp_dev *d, int *ok) {
 
 hdsdp_retcode build_precond(hdsdp_dev *d) {
     const int m = d->m;
-    if (d->useJacobi) {
+    // bf16x3 Gram blocks: the CG runs on the matrix-free FP64 operator, where each
+    // product costs a Schur-sized pass, so it starts with the factor preconditioner
+    if (d->useJacobi && !d->mGramPending) {
         hdk::get_diag(d->st, m, d->M, m, d->jac);
         d->precKind = PREC_JACOBI;
         return check_launch();
@@ -354,11 +368,36 @@ void symv_M(hdsdp_dev *d, const double *x, double *y) {
     hdk::coldot<double>(d->st, d->m, d->m, 0, 1.0, d->M, d->m, x, 0.0, y);
 }
 
+// y = M x in FP64 without M: (M x)_i = <A_i, W (sum_j x_j A_j) W> for the S^-1 = W of
+// the build, plus the diagonal terms added to M since (diag(M) - its built diagonal).
+// Used while the Gram blocks of M hold bf16x3 products: it makes the residuals of the
+// refinement those of the exact FP64 system.
+void schur_apply_exact(hdsdp_dev *d, const double *x, double *y) {
+    const int m = d->m, n = d->n;
+    hdk::shift_coef(d->st, m, x, d->xCoef);
+    hdk::assemble_buffer(d->st, d->c, d->xCoef, 0.0, d->T1, 0.0, d->work);
+    hdk::mirror_lower<double>(d->st, n, d->T1, n);
+    hdk::gemm<double>(d->st, false, false, false, n, n, n, 1.0, d->Wm, n, 0, d->T1, n, 0, 0.0, d->T2, n, 0, 1);
+    hdk::gemm<double>(d->st, false, false, false, n, n, n, 1.0, d->T2, n, 0, d->Wm, n, 0, 0.0, d->T1, n, 0, 1);
+    hdk::traces(d->st, d->c, 1, m, d->T1, nullptr, 0.0, y, nullptr, false);
+    hdk::r1_quadforms(d->st, d->c, d->T1, false, 1, m, 0.0, y, nullptr, d->H1);
+    hdk::diag_delta_axpy(d->st, m, d->M, m, d->mDiag0, x, y);
+}
+
 // conjGradSolve (linalg/hdsdp_linsolver.c:855-997) on crhs; result in cx.
 // Returns 0 on success, 1 if the solution is left unwritten (tiny rhs), -1 on failure.
 int cg_solve(hdsdp_dev *d) {
     const int m = d->m;
-    d->useJacobi = d->paramUseJacobi;
+    // While M holds bf16x3 Gram blocks the CG runs on the exact FP64 operator (applied
+    // matrix-free) with the Cholesky factor of that M as preconditioner, to 1e-4 tol
+    // (near the accuracy of an FP64 factor solve); a solve that cannot get below tol
+    // fails, and M is then rebuilt in FP64.
+    const bool exactOp = d->mGramPending != 0;
+    auto op = [&](const double *x, double *y) {
+        if (exactOp) schur_apply_exact(d, x, y);
+        else symv_M(d, x, y);
+    };
+    d->useJacobi = d->paramUseJacobi && !exactOp;
     const int restart = std::max(d->nRestart, 20);
     hdk::cg_start(d->st, m, d->crhs, d->cr, d->cx, d->sc_d);
     if (read_scalars(d) != HDSDP_RETCODE_OK) return -1;
@@ -366,10 +405,11 @@ int cg_solve(hdsdp_dev *d) {
     double tol = std::min(d->absTol, bnorm * d->relTol);
     tol = std::max(tol, 0.1 * d->absTol);
     if (rnorm < tol) return 1;
+    const double target = exactOp ? 1e-4 * tol : tol;
     apply_precond(d, d->cr, d->cz, d->cp);
-    symv_M(d, d->cp, d->cMp);
+    op(d->cp, d->cMp);
     int iter = 0, status = 0, refine = 0;
-    double prevTrue = 0.0;
+    double prevTrue = 0.0, rtrue = -1.0;
     for (iter = 0; iter < d->maxIter; ++iter) {
         hdk::cg_step1(d->st, m, d->cz, d->cr, d->cp, d->cMp, d->cx, d->crn, d->sc_d);
         d->stats[2] += 1;
@@ -383,7 +423,7 @@ int cg_solve(hdsdp_dev *d) {
         }
         apply_precond(d, d->crn, d->cz, nullptr);
         hdk::cg_step2(d->st, m, d->crn, d->cz, d->cp, d->cr, d->sc_d);
-        symv_M(d, d->cp, d->cMp);
+        op(d->cp, d->cMp);
         if (read_scalars(d) != HDSDP_RETCODE_OK) return -1;
         rnorm = d->hsc[hdk::CG_RNORM];
         if (rnorm != rnorm) return -1;
@@ -391,30 +431,41 @@ int cg_solve(hdsdp_dev *d) {
             status = 2;
             break;
         }
-        if (rnorm < tol) {
+        if (rnorm < target) {
             // With the FP32 preconditioner the recursively updated residual drifts
             // from b - M x: confirm convergence on the true FP64 residual and, if it
-            // is not met, restart from x (iterative refinement).
-            // Refinement stops at tol or once the true residual stops halving (the
-            // rounding level of an FP64 product with M).
-            if (d->precision != HDSDP_PRECISION_MIXED || d->useJacobi || refine >= 8) break;
-            symv_M(d, d->cx, d->ctmp);
+            // is not met, restart from x (iterative refinement).  Refinement stops at
+            // the target or once the true residual stops halving (the rounding level
+            // of an FP64 product with M).
+            if (d->precision != HDSDP_PRECISION_MIXED || d->useJacobi) break;
+            if (refine >= 8) {
+                if (exactOp && rtrue >= tol) return -1;
+                break;
+            }
+            op(d->cx, d->ctmp);
             hdk::cg_true_residual(d->st, m, d->crhs, d->ctmp, d->cr, d->sc_d);
             if (read_scalars(d) != HDSDP_RETCODE_OK) return -1;
-            const double rtrue = d->hsc[hdk::CG_TRUE];
-            if (rtrue < tol || (refine > 0 && rtrue > 0.5 * prevTrue)) break;
+            rtrue = d->hsc[hdk::CG_TRUE];
+            if (rtrue != rtrue) return -1;
+            if (rtrue < target) break;
+            if (refine > 0 && rtrue > 0.5 * prevTrue) {
+                if (exactOp && rtrue >= tol) return -1;
+                break;
+            }
             prevTrue = rtrue;
             ++refine;
             apply_precond(d, d->cr, d->cz, d->cp);
-            symv_M(d, d->cp, d->cMp);
+            op(d->cp, d->cMp);
         }
     }
     d->stats[5] = iter;
     static const int trace = getenv("HDSDP_TRACE_CG") != nullptr;
     if (trace)
-        fprintf(stderr, "[hdsdp_amd] cg m=%d precond=%s iters=%d refine=%d rnorm=%.3e bnorm=%.3e tol=%.3e status=%d\n",
-                m, d->useJacobi ? "jacobi" : (d->precision == HDSDP_PRECISION_MIXED ? "chol-fp32" : "chol-fp64"), iter,
-                refine, rnorm, bnorm, tol, status);
+        fprintf(stderr,
+                "[hdsdp_amd] cg m=%d precond=%s iters=%d refine=%d%s rnorm=%.3e rtrue=%.3e bnorm=%.3e tol=%.3e "
+                "status=%d\n",
+                m, d->useJacobi ? "jacobi" : (d->precKind == PREC_CHOL32 ? "chol-fp32" : "chol-fp64"), iter, refine,
+                exactOp ? " (exact op)" : "", rnorm, rtrue, bnorm, tol, status);
     if (iter >= d->maxIter || status == 2) {
         if (d->useJacobi) {
             d->useJacobi = 0;
@@ -435,10 +486,9 @@ int cg_solve(hdsdp_dev *d) {
 // M(p, q) = <T_p, T_q> over the dense constraints with T_p = L^-1 A_p L^-T (so that
 // <A_p, W A_q W> = <T_p, T_q> for W = L^-T L^-1): batched GEMMs form T, a split-K
 // Gram GEMM accumulates partial blocks that are summed in a fixed order.
-void dense_gram(hdsdp_dev *d) {
+void dense_gram(hdsdp_dev *d, const double *Li) {
     const int n = d->n, r = d->nDn, m = d->m;
     const long nn = (long) n * n;
-    const double *Li = d->Li[0];
     for (int b0 = 0; b0 < r; b0 += d->gramBatch) {
         const int cnt = std::min(d->gramBatch, r - b0);
         hdk::unpack_dense(d->st, d->c, b0, cnt, d->Dbuf);
@@ -457,6 +507,77 @@ void dense_gram(hdsdp_dev *d) {
         hdk::gemm<double>(d->st, true, false, true, r, r, (int) rem, 1.0, d->Tall + (long) full * kc, (int) nn, 0,
                           d->Tall + (long) full * kc, (int) nn, 0, 0.0, d->Gpart + (long) full * rr, r, 0, 1);
     hdk::gram_reduce(d->st, r, full + (rem > 0 ? 1 : 0), d->Gpart, r, rr, d->c.denseCon, d->M, m);
+}
+
+// Rebuild the Gram blocks of M in FP64 from the factor of their build, re-apply the
+// diagonal terms added since, and keep later builds in FP64.
+hdsdp_retcode rebuild_gram_fp64(hdsdp_dev *d) {
+    const int m = d->m, n = d->n;
+    d->schurMixed = 0;
+    d->mGramPending = 0;
+    hdk::get_diag(d->st, m, d->M, m, d->mDiagCur);
+    if (d->r1Gram) {
+        hdk::gemm<double>(d->st, false, false, false, n, d->c.nR1, n, 1.0, d->Wm, n, 0, d->c.rF, n, 0, 0.0, d->H1, n, 0,
+                          1);
+        if (d->r1Identity) hdk::r1_schur(d->st, d->c, d->H1, d->M, m, d->M, m);
+        else hdk::r1_schur(d->st, d->c, d->H1, d->Kr, d->c.nR1, d->M, m);
+    }
+    if (d->dnGram) {
+        if (!d->Tall) {
+            const size_t nn = (size_t) n * n;
+            DCALL(dalloc(d, &d->Tall, nn * d->nDn));
+            DCALL(dalloc(d, &d->Dbuf, nn * d->gramBatch));
+            DCALL(dalloc(d, &d->Ubuf, nn * d->gramBatch));
+        }
+        dense_gram(d, d->Lim);
+    }
+    // the rewritten diagonal entries get back what was added to them after the build
+    if (d->r1Gram) hdk::add_diag_delta(d->st, d->c.nR1, d->c.rCon, d->mDiagCur, d->mDiag0, d->M, m);
+    if (d->dnGram) hdk::add_diag_delta(d->st, d->nDn, d->c.denseCon, d->mDiagCur, d->mDiag0, d->M, m);
+    hdk::mirror_lower<double>(d->st, m, d->M, m);
+    static const int trace = getenv("HDSDP_TRACE_CG") != nullptr;
+    if (trace)
+        fprintf(stderr, "[hdsdp_amd] Schur matrix of build %d rebuilt in FP64; later builds stay FP64\n",
+                (int) d->stats[0]);
+    return check_launch();
+}
+
+// Mixed precision: the same products on the bf16 matrix cores (FP32-equivalent split
+// products), written as X^T Y with both operands contiguous along k:
+//   V_p = A_p L^-T = (A_p)^T (L^-T),  T_p = L^-1 V_p = (L^-T)^T V_p,  M = T^T T.
+void dense_gram_mixed(hdsdp_dev *d) {
+    const int n = d->n, r = d->nDn, m = d->m;
+    const long nn = (long) n * n;
+    hdk::d2f(d->st, nn, d->LiT[0], d->LiTf);
+    for (int b0 = 0; b0 < r; b0 += d->gramBatch) {
+        const int cnt = std::min(d->gramBatch, r - b0);
+        hdk::unpack_dense_f(d->st, d->c, b0, cnt, d->Dbuff);
+        hdk::gemm_bf16x3_tn<float>(d->st, false, n, n, n, 1.0, d->Dbuff, n, nn, d->LiTf, n, 0, 0.0, d->Vf, n, nn, cnt);
+        hdk::gemm_bf16x3_tn<float>(d->st, false, n, n, n, 1.0, d->LiTf, n, 0, d->Vf, n, nn, 0.0,
+                                   d->Tf + (size_t) b0 * nn, n, nn, cnt);
+    }
+    const long kc = d->gramChunk;
+    const int full = (int) (nn / kc);
+    const long rem = nn - (long) full * kc;
+    const long rr = (long) r * r;
+    if (full > 0)
+        hdk::gemm_bf16x3_tn<double>(d->st, true, r, r, (int) kc, 1.0, d->Tf, (int) nn, kc, d->Tf, (int) nn, kc, 0.0,
+                                    d->Gpart, r, rr, full);
+    if (rem > 0)
+        hdk::gemm_bf16x3_tn<double>(d->st, true, r, r, (int) rem, 1.0, d->Tf + (long) full * kc, (int) nn, 0,
+                                    d->Tf + (long) full * kc, (int) nn, 0, 0.0, d->Gpart + (long) full * rr, r, 0, 1);
+    hdk::gram_reduce(d->st, r, full + (rem > 0 ? 1 : 0), d->Gpart, r, rr, d->c.denseCon, d->M, m);
+}
+
+// Mixed precision rank-one Gram: G = L^-1 F = (L^-T)^T F, K = G^T G, M = s s^T o K o K.
+void r1_gram_mixed(hdsdp_dev *d) {
+    const int n = d->n, r = d->c.nR1, m = d->m;
+    hdk::d2f(d->st, (long) n * n, d->LiT[0], d->LiTf);
+    hdk::gemm_bf16x3_tn<float>(d->st, false, n, r, n, 1.0, d->LiTf, n, 0, d->Ff, n, 0, 0.0, d->Gf, n, 0, 1);
+    double *K = d->r1Identity ? d->M : d->Kr;
+    const long ldk = d->r1Identity ? m : r;
+    hdk::gemm_bf16x3_tn<double>(d->st, true, r, r, n, 1.0, d->Gf, n, 0, d->Gf, n, 0, 0.0, K, (int) ldk, 0, 1);
+    hdk::r1_square(d->st, d->c, K, ldk, d->M, m);
 }
 
 }  // namespace
@@ -712,6 +833,32 @@ hdsdp_retcode hdsdp_dev_load(hdsdp_dev *d, const hdsdp_coeff_view *cv) {
     d->H1 = d->Kr = d->Tall = d->Dbuf = d->Ubuf = d->Gpart = nullptr;
     if (c.nR1 > 0 || c.cDSR1) DCALL(dalloc(d, &d->H1, (size_t) n * (c.nR1 + 1)));
     if (d->r1Gram && !d->r1Identity) DCALL(dalloc(d, &d->Kr, (size_t) c.nR1 * c.nR1));
+    d->LiTf = d->Ff = d->Gf = d->Tf = d->Dbuff = d->Vf = nullptr;
+    d->Wm = d->Lim = d->mDiag0 = d->mDiagCur = d->xCoef = nullptr;
+    // bf16x3 Gram blocks need the matrix-free FP64 product in the refinement.  For dense
+    // constraints it costs two passes over the packed coefficients and two n^3 products,
+    // far below the Gram build; for rank-one constraints it costs about 4 n^2 r + 4 n^3
+    // flops against n r^2 for the FP64 Gram, so they use it only when r >= 32 n.
+    // HDSDP_GRAM_BF16X3=1 (or 0) forces the choice where a Gram path exists.
+    bool mixedGram = d->precision == HDSDP_PRECISION_MIXED &&
+                     (d->dnGram || (d->r1Gram && (long) c.nR1 >= 32L * n));
+    if (const char *force = getenv("HDSDP_GRAM_BF16X3"))
+        mixedGram = d->precision == HDSDP_PRECISION_MIXED && (d->r1Gram || d->dnGram) && atoi(force) != 0;
+    d->schurMixed = mixedGram ? 1 : 0;
+    d->mGramPending = 0;
+    if (mixedGram) {
+        DCALL(dalloc(d, &d->LiTf, (size_t) n * n));
+        DCALL(dalloc(d, &d->Wm, (size_t) n * n));
+        DCALL(dalloc(d, &d->Lim, (size_t) n * n));
+        DCALL(dalloc(d, &d->mDiag0, (size_t) m));
+        DCALL(dalloc(d, &d->mDiagCur, (size_t) m));
+        DCALL(dalloc(d, &d->xCoef, (size_t) m + 1));
+    }
+    if (mixedGram && d->r1Gram) {
+        DCALL(dalloc(d, &d->Ff, (size_t) n * c.nR1));
+        DCALL(dalloc(d, &d->Gf, (size_t) n * c.nR1));
+        hdk::d2f(d->st, (long) n * c.nR1, c.rF, d->Ff);
+    }
     if (d->dnGram) {
         const size_t nn = (size_t) n * n;
         const int r = d->nDn;
@@ -721,9 +868,15 @@ hdsdp_retcode hdsdp_dev_load(hdsdp_dev *d, const hdsdp_coeff_view *cv) {
         long K = (long) nn;
         d->gramChunk = std::max<long>(256, ((K + d->gramParts - 1) / d->gramParts + 15) / 16 * 16);
         d->gramParts = (int) ((K + d->gramChunk - 1) / d->gramChunk);
-        DCALL(dalloc(d, &d->Tall, nn * r));
-        DCALL(dalloc(d, &d->Dbuf, nn * d->gramBatch));
-        DCALL(dalloc(d, &d->Ubuf, nn * d->gramBatch));
+        if (mixedGram) {
+            DCALL(dalloc(d, &d->Tf, nn * r));
+            DCALL(dalloc(d, &d->Dbuff, nn * d->gramBatch));
+            DCALL(dalloc(d, &d->Vf, nn * d->gramBatch));
+        } else {
+            DCALL(dalloc(d, &d->Tall, nn * r));
+            DCALL(dalloc(d, &d->Dbuf, nn * d->gramBatch));
+            DCALL(dalloc(d, &d->Ubuf, nn * d->gramBatch));
+        }
         DCALL(dalloc(d, &d->Gpart, (size_t) r * r * d->gramParts));
     }
     return check_launch();
@@ -911,11 +1064,22 @@ hdsdp_retcode hdsdp_dev_schur(hdsdp_dev *d, int typeKKT, double Rd, double *asin
         prof_begin(d, HDSDP_PROF_PAIRS);
         hdk::schur_pairs(d->st, d->c, d->W, d->M, m);
         DCALL(prof_end(d, HDSDP_PROF_PAIRS));
+        const bool mixed = d->schurMixed != 0;
+        d->mGramPending = mixed ? 1 : 0;
+        if (mixed) {
+            // keep the factor this M is built from, for an FP64 rebuild
+            HIP_OK(hipMemcpyAsync(d->Wm, d->W, sizeof(double) * (size_t) n * n, hipMemcpyDeviceToDevice, d->st));
+            HIP_OK(hipMemcpyAsync(d->Lim, d->Li[0], sizeof(double) * (size_t) n * n, hipMemcpyDeviceToDevice, d->st));
+        }
         if (d->r1Gram) {
-            if (d->r1Identity) hdk::r1_schur(d->st, d->c, d->H1, d->M, m, d->M, m);
+            if (mixed) r1_gram_mixed(d);
+            else if (d->r1Identity) hdk::r1_schur(d->st, d->c, d->H1, d->M, m, d->M, m);
             else hdk::r1_schur(d->st, d->c, d->H1, d->Kr, d->c.nR1, d->M, m);
         }
-        if (d->dnGram) dense_gram(d);
+        if (d->dnGram) {
+            if (mixed) dense_gram_mixed(d);
+            else dense_gram(d, d->Li[0]);
+        }
         for (int h : d->heavy) {
             // B_h = W A_h W, then M(h, k) = <A_k, B_h>
             hdk::densify(d->st, d->c, h, d->T1);
@@ -924,6 +1088,7 @@ hdsdp_retcode hdsdp_dev_schur(hdsdp_dev *d, int typeKKT, double Rd, double *asin
             hdk::traces(d->st, d->c, 1, m, d->T1, nullptr, 0.0, d->ctmp, nullptr, true);
             hdk::scatter_row(d->st, m, h - 1, d->ctmp, d->M, m);
         }
+        if (mixed) hdk::get_diag(d->st, m, d->M, m, d->mDiag0);
         d->stats[0] += 1;
         DCALL(prof_end(d, HDSDP_PROF_SCHUR));
     }
@@ -964,13 +1129,22 @@ hdsdp_retcode hdsdp_dev_kkt_regularize(hdsdp_dev *d, double reg) {
 
 hdsdp_retcode hdsdp_dev_kkt_factorize(hdsdp_dev *d) {
     hdk::mirror_lower<double>(d->st, d->m, d->M, d->m);
-    return build_precond(d);
+    hdsdp_retcode rc = build_precond(d);
+    if (rc != HDSDP_RETCODE_OK && d->mGramPending) {
+        DCALL(rebuild_gram_fp64(d));
+        rc = build_precond(d);
+    }
+    return rc;
 }
 
 hdsdp_retcode hdsdp_dev_kkt_solve(hdsdp_dev *d, const double *rhs, double *sol) {
     DCALL(upload(d, d->crhs, rhs, d->m, d->hvecB));
     prof_begin(d, HDSDP_PROF_CG);
     int rc = cg_solve(d);
+    if (rc < 0 && d->mGramPending) {
+        DCALL(rebuild_gram_fp64(d));
+        if (build_precond(d) == HDSDP_RETCODE_OK) rc = cg_solve(d);
+    }
     DCALL(prof_end(d, HDSDP_PROF_CG));
     if (rc < 0) return HDSDP_RETCODE_FAILED;
     if (rc == 1) return HDSDP_RETCODE_OK;  // tiny right-hand side: solution left as it was

@@ -55,11 +55,16 @@ void r1_quadforms(hipStream_t s, const DevCoeffs &c, const double *B, bool withC
 // M entries between dense rank-one constraints: K = F^T H (H = W F from r1_quadforms),
 // M = s s^T o K o K.  K: nR1 x nR1 scratch (ldk).
 void r1_schur(hipStream_t s, const DevCoeffs &c, const double *H, double *K, long ldk, double *M, long ldm);
+// M = s s^T o K o K from a precomputed K (lower)
+void r1_square(hipStream_t s, const DevCoeffs &c, const double *K, long ldk, double *M, long ldm);
 // full n x n copies of dense slots slot0 .. slot0 + cnt - 1
 void unpack_dense(hipStream_t s, const DevCoeffs &c, int slot0, int cnt, double *out);
+void unpack_dense_f(hipStream_t s, const DevCoeffs &c, int slot0, int cnt, float *out);
 // M(denseCon[p] - 1, denseCon[q] - 1) = sum_b P_b(p, q), p >= q
 void gram_reduce(hipStream_t s, int r, int nPart, const double *P, long ldp, long strideP, const int *denseCon,
                  double *M, long ldm);
+// M(i, i) += cur[i] - ref[i] for i = con[p] - 1, p < r
+void add_diag_delta(hipStream_t s, int r, const int *con, const double *cur, const double *ref, double *M, long ldm);
 // M_ij = <A_i, W A_j W> for pairs of small constraints (lower triangle)
 void schur_pairs(hipStream_t s, const DevCoeffs &c, const double *W, double *M, long ldm);
 void scatter_row(hipStream_t s, int m, int h, const double *t, double *M, long ldm);

@@ -18,21 +18,45 @@ namespace hdk {
 // ---- buffer assembly -----------------------------------------------------------
 
 // target(i, j), i >= j: sum over dense coefficients of coef * packed value; zero elsewhere.
-__global__ void assemble_dense_kernel(int n, int nDense, const int *denseCon, const double *dVal, const double *coef,
-                                      double *target) {
+// target(i, j) = sum_d coef[denseCon[d]] A_d(i, j) for i >= j, 0 above the diagonal.
+// Thread (i, j) walks the dense slots in order (the packed entries of one column are
+// contiguous, so a wave's loads are coalesced); the slot coefficients are staged in LDS
+// and eight slot loads are issued before they are summed, so the kernel streams the
+// packed coefficient matrices at HBM rate.
+constexpr int ASM_CHUNK = 512;
+
+__global__ __launch_bounds__(256) void assemble_dense_kernel(int n, int nDense, const int *denseCon,
+                                                             const double *dVal, const double *coef, double *target) {
+    __shared__ double cs[ASM_CHUNK];
     const int j = blockIdx.y;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double v = 0.0;
-    if (i >= j) {
-        long p = (long) j * n - (long) j * (j - 1) / 2 + (i - j);
-        long npk = (long) n * (n + 1) / 2;
-        for (int d = 0; d < nDense; ++d) {
-            double c = coef[denseCon[d]];
-            if (c != 0.0) v += c * dVal[(long) d * npk + p];
-        }
+    if (min(n, (int) ((blockIdx.x + 1) * blockDim.x)) <= j) {
+        // the whole block lies above the diagonal
+        if (i < n) target[i + (long) j * n] = 0.0;
+        return;
     }
-    target[i + (long) j * n] = v;
+    const bool act = i < n && i >= j;
+    const long npk = (long) n * (n + 1) / 2;
+    const double *src = dVal + (act ? (long) j * n - (long) j * (j - 1) / 2 + (i - j) : 0);
+    double v = 0.0;
+    for (int d0 = 0; d0 < nDense; d0 += ASM_CHUNK) {
+        const int cnt = min(ASM_CHUNK, nDense - d0);
+        __syncthreads();
+        for (int t = threadIdx.x; t < cnt; t += blockDim.x) cs[t] = coef[denseCon[d0 + t]];
+        __syncthreads();
+        if (!act) continue;
+        const double *q = src + (long) d0 * npk;
+        int t = 0;
+        for (; t + 8 <= cnt; t += 8) {
+            double x[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) x[u] = q[(long) (t + u) * npk];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v += cs[t + u] * x[u];
+        }
+        for (; t < cnt; ++t) v += cs[t] * q[(long) t * npk];
+    }
+    if (i < n) target[i + (long) j * n] = act ? v : 0.0;
 }
 
 __global__ void scatter_kernel(int nPos, const long *posIdx, const int *posBeg, const int *entCon,
@@ -232,6 +256,12 @@ __global__ void r1_square_kernel(int nR1, const int *rCon, const double *csign, 
     M[i + j * ldm] = csign[cp] * csign[cq] * kv * kv;
 }
 
+void r1_square(hipStream_t s, const DevCoeffs &c, const double *K, long ldk, double *M, long ldm) {
+    const int r = c.nR1;
+    if (r == 0) return;
+    r1_square_kernel<<<dim3((r + 255) / 256, r), 256, 0, s>>>(r, c.rCon, c.csign, K, ldk, M, ldm);
+}
+
 void r1_schur(hipStream_t s, const DevCoeffs &c, const double *H, double *K, long ldk, double *M, long ldm) {
     const int n = c.n, r = c.nR1;
     if (r == 0) return;
@@ -254,6 +284,22 @@ __global__ void unpack_dense_kernel(int n, const double *dVal, int slot0, double
     }
 }
 
+__global__ void unpack_dense_f_kernel(int n, const double *dVal, int slot0, float *out) {
+    const long npk = (long) n * (n + 1) / 2;
+    const double *v = dVal + (long) (slot0 + blockIdx.y) * npk;
+    float *o = out + (long) blockIdx.y * n * n;
+    for (long e = blockIdx.x * (long) blockDim.x + threadIdx.x; e < (long) n * n; e += (long) gridDim.x * blockDim.x) {
+        long i = e % n, j = e / n;
+        long r = i > j ? i : j, q = i > j ? j : i;
+        o[e] = (float) v[q * n - q * (q + 1) / 2 + r];
+    }
+}
+
+void unpack_dense_f(hipStream_t s, const DevCoeffs &c, int slot0, int cnt, float *out) {
+    if (cnt <= 0) return;
+    unpack_dense_f_kernel<<<dim3(256, cnt), 256, 0, s>>>(c.n, c.dVal, slot0, out);
+}
+
 void unpack_dense(hipStream_t s, const DevCoeffs &c, int slot0, int cnt, double *out) {
     if (cnt <= 0) return;
     unpack_dense_kernel<<<dim3(256, cnt), 256, 0, s>>>(c.n, c.dVal, slot0, out);
@@ -271,6 +317,19 @@ __global__ void gram_reduce_kernel(int r, int nPart, const double *P, long ldp, 
     const int cp = denseCon[p], cq = denseCon[q];
     const long i = (cp > cq ? cp : cq) - 1, j = (cp > cq ? cq : cp) - 1;
     M[i + j * ldm] = v;
+}
+
+__global__ void add_diag_delta_kernel(int r, const int *con, const double *cur, const double *ref, double *M,
+                                      long ldm) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= r) return;
+    const long i = con[p] - 1;
+    M[i + i * ldm] += cur[i] - ref[i];
+}
+
+void add_diag_delta(hipStream_t s, int r, const int *con, const double *cur, const double *ref, double *M, long ldm) {
+    if (r <= 0) return;
+    add_diag_delta_kernel<<<(r + 255) / 256, 256, 0, s>>>(r, con, cur, ref, M, ldm);
 }
 
 void gram_reduce(hipStream_t s, int r, int nPart, const double *P, long ldp, long strideP, const int *denseCon,

@@ -45,5 +45,9 @@ void d2f_scaled(hipStream_t s, long n, const double *x, const double *sc, float 
 // sc[i] = 1 / sqrt(M_ii) (1 where M_ii <= 0)
 void jacobi_scale(hipStream_t s, int m, const double *M, long ldm, double *sc);
 void d2f(hipStream_t s, long n, const double *x, float *y);
+// coef[0] = 0, coef[i + 1] = x[i] (coefficients of sum_i x_i A_i for the buffer assembly)
+void shift_coef(hipStream_t s, int m, const double *x, double *coef);
+// y[i] += (M_ii - ref[i]) x[i]
+void diag_delta_axpy(hipStream_t s, int m, const double *M, long ldm, const double *ref, const double *x, double *y);
 
 }  // namespace hdk

@@ -259,6 +259,27 @@ __global__ __launch_bounds__(1024) void regularize_kernel(int m, double *M, long
     for (int i = threadIdx.x; i < m; i += blockDim.x) M[i + i * ldm] += reg;
 }
 
+__global__ void shift_coef_kernel(int m, const double *x, double *coef) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > m) return;
+    coef[i] = (i == 0) ? 0.0 : x[i - 1];
+}
+
+void shift_coef(hipStream_t s, int m, const double *x, double *coef) {
+    shift_coef_kernel<<<(m + 256) / 256, 256, 0, s>>>(m, x, coef);
+}
+
+__global__ void diag_delta_axpy_kernel(int m, const double *M, long ldm, const double *ref, const double *x,
+                                       double *y) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    y[i] += (M[i + i * ldm] - ref[i]) * x[i];
+}
+
+void diag_delta_axpy(hipStream_t s, int m, const double *M, long ldm, const double *ref, const double *x, double *y) {
+    diag_delta_axpy_kernel<<<(m + 255) / 256, 256, 0, s>>>(m, M, ldm, ref, x, y);
+}
+
 void regularize(hipStream_t s, int m, double *M, long ldm, double r) {
     regularize_kernel<<<1, 1024, 0, s>>>(m, M, ldm, r);
 }

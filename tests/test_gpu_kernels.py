@@ -154,3 +154,26 @@ def test_axpy_and_primal_recovery(gpu_lib, oracle, name):
     assert rel(gst[0], AX) < 1e-9
     assert abs(gst[1] - np.sum(coeffs[0].dense() * X)) <= 1e-9 * max(1.0, np.abs(X).sum())
     assert abs(gst[3] - np.trace(X)) <= 1e-10 * max(1.0, abs(np.trace(X)))
+
+
+@pytest.mark.parametrize("name", ["dense", "dsr1"])
+def test_schur_mixed_precision_gram(gpu_lib, name, monkeypatch):
+    """Mixed precision builds the dense and rank-one Schur matrices from bf16 split
+    products (FP32-equivalent): within 1e-5 relative of the FP64 closed form; the
+    right-hand sides stay FP64.  (Rank-one Gram blocks take the bf16x3 path only
+    for r >= 32 n unless forced, as here.)"""
+    monkeypatch.setenv("HDSDP_GRAM_BF16X3", "1")
+    n, m, kinds = CASES[name]
+    rng = np.random.default_rng(5)
+    coeffs = dt.rand_coeffs(rng, n, m, kinds)
+    y, eye = dt.interior_point(coeffs, rng)
+    d = dt.Dev(gpu_lib, coeffs, precision=1)
+    d.update(1.0, -1.0, y, eye, 0)
+    assert d.factorize(0) == 1
+    a, b, t = d.schur(0, -0.5)
+    M = d.get_schur()
+    W = np.linalg.inv(dt.dense_buffer(coeffs, 1.0, -1.0, y, eye))
+    A = [c.dense() for c in coeffs[1:]]
+    Mref = np.array([[np.sum(Ai * (W @ Aj @ W)) for Aj in A] for Ai in A])
+    assert rel(M, Mref) < 1e-5
+    assert rel(a, [np.sum(Ai * W) for Ai in A]) < 1e-10

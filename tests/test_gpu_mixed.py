@@ -1,35 +1,62 @@
-"""Mixed precision (FP32 Cholesky of the Schur matrix as the preconditioner of
-the FP64 CG) against the plain FP64 GPU path on the same instances.  The CG
-still stops on FP64 residuals at the reference's tolerances, so the iterates
-agree to the solve tolerance: same status and iteration count, objectives
-within 1e-8 relative and y* within 1e-7 relative at the default optimality
-tolerance."""
+"""Mixed precision against the plain FP64 GPU path on the same instances, solved
+to a 1e-6 relative gap.
+
+Two kinds of instance:
+  * sparse constraints (MaxCut, theta), and rank-one constraints with r < 32 n:
+    M is built in FP64 and only its preconditioner is FP32 (equilibrated
+    Cholesky); the CG stops on FP64 residuals at the reference's tolerances,
+    so the iterates agree to the solve tolerance: same status and iteration
+    count;
+  * dense constraints (and rank-one ones when forced onto the bf16x3 Gram
+    path): the Gram blocks of M come from bf16x3 products (FP32-equivalent,
+    FP64 accumulation) and each Newton system is refined against the exact
+    FP64 Schur operator applied matrix-free, so the directions carry FP64
+    accuracy; when such an M cannot be factored or solved, or the refinement
+    stalls, the library rebuilds M in FP64 and stays there.  The trajectory
+    may still differ from FP64 by an iteration.
+All must end optimal with objectives within 1e-8 relative of the FP64 run and
+y* within 1e-7, except the rank-one instance forced onto the bf16x3 path:
+there the CG runs on the exact operator from the first iteration, where the
+FP64 path starts with Jacobi-CG solves accurate to 1e-6, and the two end
+5e-7 apart in y (measured: FP64 GPU vs reference HDSDP 3.8e-8, forced
+bf16x3 vs reference 5.1e-7); its bound is 1e-6."""
 import numpy as np
 import pytest
 
-import devtools as dt
 from hdsdp_amd import instances as I
 from hdsdp_amd.solver import run_solver
 
 pytestmark = pytest.mark.gpu
 
+# name -> (instance, Gram-built Schur matrix, HDSDP_GRAM_BF16X3 override or None, y tolerance)
 CASES = {
-    "maxcut_n200": lambda: I.maxcut(200, 0.05, 1, "pm1"),
-    "theta_n50": lambda: I.lovasz_theta(50, 0.3, 21),
-    "dense_n40_m60": lambda: I.dense_random(40, 60, 24),
-    "rankone_n60_m500": lambda: I.rank_one(60, 500, 5, 45),
+    "maxcut_n200": (lambda: I.maxcut(200, 0.05, 1, "pm1"), False, None, 1e-7),
+    "theta_n50": (lambda: I.lovasz_theta(50, 0.3, 21), False, None, 1e-7),
+    "dense_n40_m60": (lambda: I.dense_random(40, 60, 24), True, None, 1e-7),
+    "dense_n120_m240": (lambda: I.dense_random(120, 240, 33), True, None, 1e-7),
+    "rankone_n60_m500": (lambda: I.rank_one(60, 500, 5, 45), False, None, 1e-7),
+    "rankone_n200_m2000": (lambda: I.rank_one(200, 2000, 20, 4), False, None, 1e-7),
+    "rankone_n200_m2000_bf16x3": (lambda: I.rank_one(200, 2000, 20, 4), True, "1", 1e-6),
 }
 
 
 @pytest.mark.parametrize("name", list(CASES))
-def test_mixed_matches_fp64(gpu_lib, name):
-    inst = CASES[name]()
-    r64 = run_solver(gpu_lib, inst, precision=0)
-    rmx = run_solver(gpu_lib, inst, precision=1)
+def test_mixed_matches_fp64(gpu_lib, name, monkeypatch):
+    make, gram, force, ytol = CASES[name]
+    if force is not None:
+        monkeypatch.setenv("HDSDP_GRAM_BF16X3", force)
+    inst = make()
+    opts = dict(relopt=1e-6, absopt=1e30)
+    r64 = run_solver(gpu_lib, inst, precision=0, **opts)
+    rmx = run_solver(gpu_lib, inst, precision=1, **opts)
     rel_y = np.linalg.norm(rmx.y - r64.y) / np.linalg.norm(r64.y)
-    print(name, r64.status, r64.iters, rmx.status, rmx.iters, rel_y, flush=True)
+    print(name, r64.status, r64.iters, rmx.status, rmx.iters, rel_y, rmx.rel_gap, flush=True)
     assert rmx.status == r64.status == 3
-    assert rmx.iters == r64.iters
+    assert rmx.rel_gap <= 1e-6
+    if gram:
+        assert abs(rmx.iters - r64.iters) <= 2
+    else:
+        assert rmx.iters == r64.iters
     assert abs(rmx.dobj - r64.dobj) <= 1e-8 * max(1.0, abs(r64.dobj))
     assert abs(rmx.pobj - r64.pobj) <= 1e-8 * max(1.0, abs(r64.pobj))
-    assert rel_y <= 1e-7
+    assert rel_y <= ytol
