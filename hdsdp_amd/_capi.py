@@ -22,6 +22,10 @@ _API = {
     "HDSDPSetVerbose": (None, [P_VOID, C.c_int]),
     "HDSDPSetProfiling": (None, [P_VOID, C.c_int]),
     "HDSDPGetProfile": (C.c_int, [P_VOID, P_DBL, P_DBL, C.c_int]),
+    "HDSDPSetup": (C.c_int, [P_VOID]),
+    "HDSDPGetBarrierPoint": (C.c_int, [P_VOID, P_DBL, P_DBL]),
+    "HDSDPCenter": (C.c_int, [P_VOID, C.c_double, P_DBL, C.c_int, C.c_double, P_INT, P_DBL]),
+    "HDSDPAdjoint": (C.c_int, [P_VOID, C.c_double, P_DBL, P_DBL, P_DBL, P_DBL]),
     "HDSDPGetRowDual": (C.c_int, [P_VOID, P_DBL, P_DBL, P_DBL]),
     "HDSDPGetConeValues": (None, [P_VOID, C.c_int, P_DBL, P_DBL, P_DBL]),
     "HDSDPCheckSolution": (C.c_int, [P_VOID, P_DBL]),
@@ -52,6 +56,7 @@ _API = {
     "hdsdp_dev_kkt_regularize": (C.c_int, [P_VOID, C.c_double]),
     "hdsdp_dev_kkt_factorize": (C.c_int, [P_VOID]),
     "hdsdp_dev_kkt_solve": (C.c_int, [P_VOID, P_DBL, P_DBL]),
+    "hdsdp_dev_kkt_solve_exact": (C.c_int, [P_VOID, P_DBL, P_DBL]),
     "hdsdp_dev_get_schur": (C.c_int, [P_VOID, P_DBL]),
     "hdsdp_dev_primal": (C.c_int, [P_VOID, C.c_double, P_DBL, P_DBL, P_INT]),
     "hdsdp_dev_primal_stats": (C.c_int, [P_VOID, P_DBL, P_DBL, P_DBL, P_DBL]),

@@ -371,7 +371,11 @@ __global__ __launch_bounds__(64) void chol_leaf_kernel(T *A, int lda, int k0, in
 // s (so that the next leaf and panel can start at once), the rest on s2; the next
 // block column's own update waits for s2's previous update through an event.
 template <typename T>
-void potrf(hipStream_t s, int n, T *A, int lda, T *work, int *info, const LookAhead *la) {
+bool potrf(hipStream_t s, int n, T *A, int lda, T *work, size_t workLen, int *info, const LookAhead *la) {
+    if (workLen < POTRF_WORK) {
+        fprintf(stderr, "[hdsdp_amd] potrf: work of %zu elements, %zu needed\n", workLen, POTRF_WORK);
+        return false;
+    }
     hipMemsetAsync(info, 0, sizeof(int), s);
     auto at = [&](int i, int j) { return A + i + (long) j * lda; };
     auto leaf = [&](int k0) {
@@ -389,7 +393,7 @@ void potrf(hipStream_t s, int n, T *A, int lda, T *work, int *info, const LookAh
                 gemm<T>(s, false, true, true, n - k1, n - k1, NB, (T) -1, at(k1, k0), lda, 0, at(k1, k0), lda, 0,
                         (T) 1, at(k1, k1), lda, 0, 1);
         }
-        return;
+        return true;
     }
     int ring = 0;
     bool pending = false;  // an update of s2 that the next block column must wait for
@@ -421,10 +425,11 @@ void potrf(hipStream_t s, int n, T *A, int lda, T *work, int *info, const LookAh
         leaf(k1);
     }
     if (pending) hipStreamWaitEvent(s, evB, 0);
+    return true;
 }
 
-template void potrf<double>(hipStream_t, int, double *, int, double *, int *, const LookAhead *);
-template void potrf<float>(hipStream_t, int, float *, int, float *, int *, const LookAhead *);
+template bool potrf<double>(hipStream_t, int, double *, int, double *, size_t, int *, const LookAhead *);
+template bool potrf<float>(hipStream_t, int, float *, int, float *, size_t, int *, const LookAhead *);
 
 // ---- triangular inverse ------------------------------------------------------
 

@@ -48,11 +48,15 @@ struct LookAhead {
 };
 
 // In-place lower Cholesky of the n x n matrix A (lda), blocked by NB.
-// work: NB*NB scratch; info: device int, 0 on success else failing column + 1.
-// With la, the bulk of each trailing update runs on la->s2 while the next block
-// column is factored on s; s has waited for s2 when the call returns.
+// work: scratch of workLen >= POTRF_WORK elements (the leaf writes the NB x NB inverse
+// of every diagonal block, padded with the identity, whatever n is); info: device int,
+// 0 on success else failing column + 1.  Returns false, launching nothing, when
+// workLen is too small.  With la, the bulk of each trailing update runs on la->s2
+// while the next block column is factored on s; s has waited for s2 when the call
+// returns.
+constexpr size_t POTRF_WORK = (size_t) NB * NB;
 template <typename T>
-void potrf(hipStream_t s, int n, T *A, int lda, T *work, int *info, const LookAhead *la = nullptr);
+bool potrf(hipStream_t s, int n, T *A, int lda, T *work, size_t workLen, int *info, const LookAhead *la = nullptr);
 
 // Linv = L^-1 for the lower-triangular n x n factor L (upper of Linv zeroed).
 // work: n * n scratch.

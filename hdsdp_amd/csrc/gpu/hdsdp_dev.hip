@@ -271,10 +271,12 @@ hdsdp_retcode build_chol64(hdsdp_dev *d) {
     if (!d->Mchol) {
         DCALL(dalloc(d, &d->Mchol, (size_t) m * m));
         DCALL(dalloc(d, &d->MLi, (size_t) m * m));
-        DCALL(dalloc(d, &d->MLiT, (size_t) m * m));
+        DCALL(dalloc(d, &d->MLiT, std::max((size_t) m * m, hdk::POTRF_WORK)));
     }
     HIP_OK(hipMemcpyAsync(d->Mchol, d->M, sizeof(double) * (size_t) m * m, hipMemcpyDeviceToDevice, d->st));
-    hdk::potrf<double>(d->st, m, d->Mchol, m, d->MLiT, d->info_d, &d->la);
+    if (!hdk::potrf<double>(d->st, m, d->Mchol, m, d->MLiT, std::max((size_t) m * m, hdk::POTRF_WORK), d->info_d,
+                            &d->la))
+        return HDSDP_RETCODE_FAILED;
     HIP_OK(hipMemcpyAsync(d->hinfo, d->info_d, sizeof(int), hipMemcpyDeviceToHost, d->st));
     DCALL(sync(d));
     if (*d->hinfo != 0) return HDSDP_RETCODE_FAILED;
@@ -296,7 +298,7 @@ hdsdp_retcode build_chol32(hdsdp_dev *d, int *ok) {
         DCALL(dalloc(d, &d->MLiTf, (size_t) m * m));
         DCALL(dalloc(d, &d->cf1, (size_t) m));
         DCALL(dalloc(d, &d->cf2, (size_t) m));
-        DCALL(dalloc(d, &d->workf, (size_t) m * m));
+        DCALL(dalloc(d, &d->workf, std::max((size_t) m * m, hdk::POTRF_WORK)));
         DCALL(dalloc(d, &d->precS, (size_t) m));
     }
     *ok = 0;
@@ -307,7 +309,9 @@ hdsdp_retcode build_chol32(hdsdp_dev *d, int *ok) {
     for (double shift : shifts) {
         if (shift < d->precShift) continue;
         hdk::convert_lower_scaled(d->st, m, d->M, m, d->precS, shift, d->Mf, m);
-        hdk::potrf<float>(d->st, m, d->Mf, m, d->workf, d->info_d, &d->la);
+        if (!hdk::potrf<float>(d->st, m, d->Mf, m, d->workf, std::max((size_t) m * m, hdk::POTRF_WORK), d->info_d,
+                               &d->la))
+            return HDSDP_RETCODE_FAILED;
         HIP_OK(hipMemcpyAsync(d->hinfo, d->info_d, sizeof(int), hipMemcpyDeviceToHost, d->st));
         DCALL(sync(d));
         if (*d->hinfo == 0) {
@@ -612,7 +616,7 @@ hdsdp_retcode hdsdp_dev_create(hdsdp_dev **pdev, int nRow, int nCol, int precisi
     DCALL(dalloc(d, &d->W, nn));
     DCALL(dalloc(d, &d->W2, nn));
     DCALL(dalloc(d, &d->X, nn));
-    DCALL(dalloc(d, &d->T1, nn));
+    DCALL(dalloc(d, &d->T1, std::max(nn, hdk::POTRF_WORK)));
     DCALL(dalloc(d, &d->T2, nn));
     DCALL(dalloc(d, &d->coef_d, m + 1));
     DCALL(dalloc(d, &d->vecA_d, m + 1));
@@ -999,7 +1003,9 @@ hdsdp_retcode hdsdp_dev_factorize(hdsdp_dev *d, int which, int *isPsd) {
     const int n = d->n, f = fidx(which);
     prof_begin(d, HDSDP_PROF_SFACTOR);
     HIP_OK(hipMemcpyAsync(d->L[f], d->S[f], sizeof(double) * (size_t) n * n, hipMemcpyDeviceToDevice, d->st));
-    hdk::potrf<double>(d->st, n, d->L[f], n, d->T1, d->info_d, &d->la);
+    if (!hdk::potrf<double>(d->st, n, d->L[f], n, d->T1, std::max((size_t) n * n, hdk::POTRF_WORK), d->info_d,
+                            &d->la))
+        return HDSDP_RETCODE_FAILED;
     HIP_OK(hipMemcpyAsync(d->hinfo, d->info_d, sizeof(int), hipMemcpyDeviceToHost, d->st));
     DCALL(prof_end(d, HDSDP_PROF_SFACTOR));
     DCALL(sync(d));
@@ -1151,6 +1157,28 @@ hdsdp_retcode hdsdp_dev_kkt_solve(hdsdp_dev *d, const double *rhs, double *sol) 
     return download(d, sol, d->cx, d->m, d->hvecB);
 }
 
+// Direct FP64 solve of M x = rhs (FP64 Gram blocks, FP64 Cholesky, two refinement steps
+// on the residual with M), for the centring and adjoint solves, which need the solution
+// to working accuracy rather than to the interior-point tolerance.
+hdsdp_retcode hdsdp_dev_kkt_solve_exact(hdsdp_dev *d, const double *rhs, double *sol) {
+    const int m = d->m;
+    if (d->mGramPending) DCALL(rebuild_gram_fp64(d));
+    hdk::mirror_lower<double>(d->st, m, d->M, m);
+    prof_begin(d, HDSDP_PROF_MFACTOR);
+    DCALL(build_chol64(d));
+    DCALL(prof_end(d, HDSDP_PROF_MFACTOR));
+    DCALL(upload(d, d->crhs, rhs, m, d->hvecB));
+    apply_precond(d, d->crhs, d->cx, nullptr);
+    for (int it = 0; it < 2; ++it) {
+        symv_M(d, d->cx, d->ctmp);
+        hdk::cg_residual(d->st, m, d->crhs, d->ctmp, d->cr);
+        apply_precond(d, d->cr, d->cz, nullptr);
+        hdk::axpy(d->st, m, 1.0, d->cz, d->cx);
+    }
+    DCALL(check_launch());
+    return download(d, sol, d->cx, m, d->hvecB);
+}
+
 hdsdp_retcode hdsdp_dev_get_schur(hdsdp_dev *d, double *M) {
     const int m = d->m;
     hdk::mirror_lower<double>(d->st, m, d->M, m);
@@ -1216,17 +1244,20 @@ hdsdp_retcode hdsdp_dev_get_dual(hdsdp_dev *d, double *S) {
 hdsdp_retcode hdsdp_dev_adjoint(hdsdp_dev *d, double mu, const double *dLdy, double *dLdb, double *dLdC) {
     const int m = d->m, n = d->n;
     std::vector<double> lam(m, 0.0);
-    DCALL(hdsdp_dev_kkt_solve(d, dLdy, lam.data()));
+    DCALL(hdsdp_dev_kkt_solve_exact(d, dLdy, lam.data()));
     for (int i = 0; i < m; ++i) dLdb[i] = lam[i] / mu;
-    // dLdC = -W (sum_i lambda_i A_i) W with W = S^-1 of the dual factor
+    // dLdC = W (sum_i lambda_i A_i) W with W = S^-1 of the dual factor
     DCALL(ensure_linv(d, 0));
     hdk::gemm<double>(d->st, true, false, false, n, n, n, 1.0, d->Li[0], n, 0, d->Li[0], n, 0, 0.0, d->W, n, 0, 1);
     DCALL(hdsdp_dev_update_buffer(d, 0.0, 1.0, lam.data(), 0.0, 2));
     hdk::mirror_lower<double>(d->st, n, d->S[2], n);
     hdk::gemm<double>(d->st, false, false, false, n, n, n, 1.0, d->W, n, 0, d->S[2], n, 0, 0.0, d->T1, n, 0, 1);
-    hdk::gemm<double>(d->st, false, false, false, n, n, n, -1.0, d->T1, n, 0, d->W, n, 0, 0.0, d->T2, n, 0, 1);
+    hdk::gemm<double>(d->st, false, false, false, n, n, n, 1.0, d->T1, n, 0, d->W, n, 0, 0.0, d->T2, n, 0, 1);
+    // the gradient with respect to a symmetric matrix: (G + G^T) / 2 (the two products
+    // leave rounding-level asymmetry, large in absolute terms when S is near-singular)
+    hdk::sym_avg(d->st, n, d->T2, 1.0, 0.0, d->T1);
     DCALL(sync(d));
-    HIP_OK(hipMemcpy(dLdC, d->T2, sizeof(double) * (size_t) n * n, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(dLdC, d->T1, sizeof(double) * (size_t) n * n, hipMemcpyDeviceToHost));
     return HDSDP_RETCODE_OK;
 }
 

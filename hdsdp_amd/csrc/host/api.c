@@ -359,17 +359,10 @@ int HDSDPGetProfile( hdsdp *s, double *ms, double *counts, int nSlots ) {
 }
 
 /* interface/hdsdp.c:617-689 */
-hdsdp_retcode HDSDPOptimize( hdsdp *s, int dOptOnly ) {
+/* Pre-solve: cone and device set-up, rank-one detection, statistics, scaling and
+ * parameter adjustment, bound cone (interface/hdsdp.c:617-689). */
+static hdsdp_retcode presolve( hdsdp *s ) {
     hdsdp_retcode retcode = HDSDP_RETCODE_OK;
-    hlog(s, "\nHDSDP: software for semi-definite programming (MI355X build, backend %s)\n",
-         hdsdp_dev_backend_name());
-    hlog(s, "---------------------------------------------\n");
-    s->dTimeBegin = hdsdp_timestamp();
-    s->nLog = 0;
-    if ( !s->userCones || !s->userCones[0] ) {
-        retcode = HDSDP_RETCODE_FAILED;
-        goto exit_cleanup;
-    }
     hlog(s, "Pre-solver starts \n");
     HCALL(sdp_cone_create(&s->sdp, s->userCones[0], s->precision));
     if ( s->profiling ) HCALL(hdsdp_dev_set_profiling(s->sdp->dev, 1));
@@ -383,6 +376,30 @@ hdsdp_retcode HDSDPOptimize( hdsdp *s, int dOptOnly ) {
     hlog(s, "    Dual box up : %+5.1e\n", get_dbl_param(s, DBL_PARAM_DUALBOX_UP));
     hlog(s, "Pre-solver ends. Elapsed time: %3.1f seconds \n", hdsdp_timestamp() - s->dTimeBegin);
     hlog(s, "%s", s->modelFeatures);
+exit_cleanup:
+    return retcode;
+}
+
+hdsdp_retcode HDSDPSetup( hdsdp *s ) {
+    if ( !s->userCones || !s->userCones[0] ) return HDSDP_RETCODE_FAILED;
+    if ( s->sdp ) return HDSDP_RETCODE_OK;
+    s->dTimeBegin = hdsdp_timestamp();
+    return presolve(s);
+}
+
+hdsdp_retcode HDSDPOptimize( hdsdp *s, int dOptOnly ) {
+    hdsdp_retcode retcode = HDSDP_RETCODE_OK;
+    hlog(s, "\nHDSDP: software for semi-definite programming (MI355X build, backend %s)\n",
+         hdsdp_dev_backend_name());
+    hlog(s, "---------------------------------------------\n");
+    s->dTimeBegin = hdsdp_timestamp();
+    s->nLog = 0;
+    if ( !s->userCones || !s->userCones[0] ) {
+        retcode = HDSDP_RETCODE_FAILED;
+        goto exit_cleanup;
+    }
+    if ( !s->sdp ) HCALL(presolve(s));
+    s->adjReady = 0;
 
     retcode = hdsdp_solve(s, dOptOnly);
     s->dTimeEnd = hdsdp_timestamp();
@@ -511,6 +528,7 @@ void HDSDPClear( hdsdp *s ) {
     free(s->dAccRowDualMaker); free(s->dAccRowDualStepMaker);
     free(s->dInaccRowDualMaker); free(s->dInaccRowDualStepMaker);
     free(s->iterLog);
+    free(s->adjY);
     int verbose = s->verbose, profiling = s->profiling;
     memset(s, 0, sizeof(hdsdp));
     s->verbose = verbose;

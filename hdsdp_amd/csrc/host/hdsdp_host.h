@@ -196,6 +196,12 @@ struct hdsdp_solver_internal {
     double dTimeEnd;
     hdsdp_status HStatus;
 
+    /* centring / adjoint: the point (mu, y) whose dual factor and Schur matrix
+     * the device holds (adjoint.c) */
+    int adjReady;
+    double adjMu;
+    double *adjY;
+
     int HIntParams[NUM_INT_PARAM];
     double HDblParams[NUM_DBL_PARAM];
     int HIntFeatures[20];

@@ -27,6 +27,7 @@ __all__ = [
     "rank_one",
     "write_instance",
     "read_instance",
+    "with_data",
     "CONFIGS",
 ]
 
@@ -103,6 +104,29 @@ def _random_graph(rng, n, p):
     iu, ju = np.triu_indices(n, 1)  # i < j
     keep = rng.random(iu.shape[0]) < p
     return iu[keep], ju[keep]
+
+
+def with_data(inst: SDPInstance, b=None, C=None) -> SDPInstance:
+    """``inst`` with the right-hand side ``b`` and/or the objective ``C`` (dense
+    symmetric ``n x n``; its lower triangle is stored, zeros dropped) replaced."""
+    out = dataclasses.replace(inst)
+    if b is not None:
+        out.b = np.array(b, dtype=np.float64).reshape(inst.m)
+    if C is not None:
+        n = inst.n
+        Cm = np.asarray(C, dtype=np.float64).reshape(n, n)
+        jj, ii = np.nonzero(np.tril(Cm).T)  # column-major order of the lower triangle
+        vals = Cm[ii, jj]
+        cpk = packed_index(n, ii, jj)
+        order = np.argsort(cpk, kind="stable")
+        c_idx, c_val = cpk[order].astype(np.int32), vals[order]
+        lo, hi = int(inst.beg[1]), int(inst.beg[-1])
+        shift = c_idx.shape[0] - lo
+        out.beg = inst.beg.copy()
+        out.beg[1:] += shift
+        out.idx = np.concatenate([c_idx, inst.idx[lo:hi]])
+        out.elem = np.concatenate([c_val, inst.elem[lo:hi]])
+    return out
 
 
 def maxcut_graph(n: int, i, j, w, name: str = "maxcut") -> SDPInstance:

@@ -71,83 +71,141 @@ class SolveResult:
         return (self.pobj - self.dobj) / (abs(self.pobj) + abs(self.dobj) + 1.0)
 
 
-def run_solver(lib: C.CDLL, inst: SDPInstance, *, psdp: int = 0, relopt: float | None = None,
-               absopt: float | None = None, maxiter: int | None = None,
-               timelimit: float | None = None, precision: int = 0, verbose: bool = False,
-               gap_target: float = 1e-6, profile: bool = False, keep=None) -> SolveResult:
-    """Solve ``inst`` through the C API of ``lib``.  ``keep`` (a list) receives the
-    solver handle instead of destroying it, for callers that query it further."""
-    m, n = inst.m, inst.n
-    solver = C.c_void_p()
-    ud = C.c_void_p()
-    beg32 = np.ascontiguousarray(inst.beg, dtype=np.int32)
-    idx = np.ascontiguousarray(inst.idx, dtype=np.int32)
-    elem = np.ascontiguousarray(inst.elem, dtype=np.float64)
-    b = np.ascontiguousarray(inst.b, dtype=np.float64)
-    arrays = [beg32, idx, elem, b]
-    if lib.HUserDataCreate(C.byref(ud)):
-        raise RuntimeError("HUserDataCreate failed")
-    if inst.factor is not None:
-        fac = np.asfortranarray(inst.factor, dtype=np.float64)
-        sgn = np.ascontiguousarray(inst.sign, dtype=np.float64)
-        arrays += [fac, sgn]
-        nnz_c = int(inst.beg[1])
-        lib.HUserDataSetRankOneConeData(ud, m, n, nnz_c, _capi.iptr(idx), _capi.dptr(elem),
-                                        fac.ctypes.data_as(_capi.P_DBL), _capi.dptr(sgn))
-    else:
-        if inst.nnz > np.iinfo(np.int32).max:
-            raise ValueError("instance exceeds the int32 CSC interface")
-        lib.HUserDataSetConeData(ud, 4, m, n, _capi.iptr(beg32), _capi.iptr(idx), _capi.dptr(elem))
-    if lib.HDSDPCreate(C.byref(solver)):
-        raise RuntimeError("HDSDPCreate failed")
-    try:
-        if lib.HDSDPInit(solver, m, 1):
-            raise RuntimeError("HDSDPInit failed")
-        lib.HDSDPSetVerbose(solver, 1 if verbose else 0)
-        if lib.HDSDPSetCone(solver, 0, ud):
-            raise RuntimeError("HDSDPSetCone failed")
-        lib.HDSDPSetDualObjective(solver, _capi.dptr(b))
-        lib.HDSDPSetIntParamAny(solver, INT_PARAM["psdp"], int(psdp))
-        if relopt is not None:
-            lib.HDSDPSetDblParam(solver, DBL_PARAM["relopt"], float(relopt))
-        if absopt is not None:
-            lib.HDSDPSetDblParam(solver, DBL_PARAM["absopt"], float(absopt))
-        if maxiter is not None:
-            lib.HDSDPSetIntParam(solver, INT_PARAM["maxiter"], int(maxiter))
-        if timelimit is not None:
-            lib.HDSDPSetDblParam(solver, DBL_PARAM["timelimit"], float(timelimit))
-        lib.HDSDPSetPrecision(solver, int(precision))
-        lib.HDSDPSetProfiling(solver, 1 if profile else 0)
-        rc = lib.HDSDPOptimize(solver, 1)
+class Solver:
+    """One solver handle on ``inst`` through the C API of ``lib`` (create, set the
+    cone, parameters; ``close`` destroys it).  The arrays handed to the C side are
+    kept alive here: the user data stores the caller's pointers without copying
+    (``interface/hdsdp_user_data.c:60-71``)."""
+
+    def __init__(self, lib: C.CDLL, inst: SDPInstance, *, psdp: int = 0, relopt: float | None = None,
+                 absopt: float | None = None, maxiter: int | None = None, timelimit: float | None = None,
+                 precision: int = 0, verbose: bool = False, profile: bool = False):
+        self.lib, self.inst = lib, inst
+        self.m, self.n = m, n = inst.m, inst.n
+        self.profile = profile
+        self.h = C.c_void_p()
+        self.ud = C.c_void_p()
+        beg32 = np.ascontiguousarray(inst.beg, dtype=np.int32)
+        idx = np.ascontiguousarray(inst.idx, dtype=np.int32)
+        elem = np.ascontiguousarray(inst.elem, dtype=np.float64)
+        b = np.ascontiguousarray(inst.b, dtype=np.float64)
+        self._arrays = [beg32, idx, elem, b]
+        if lib.HUserDataCreate(C.byref(self.ud)):
+            raise RuntimeError("HUserDataCreate failed")
+        if inst.factor is not None:
+            fac = np.asfortranarray(inst.factor, dtype=np.float64)
+            sgn = np.ascontiguousarray(inst.sign, dtype=np.float64)
+            self._arrays += [fac, sgn]
+            nnz_c = int(inst.beg[1])
+            lib.HUserDataSetRankOneConeData(self.ud, m, n, nnz_c, _capi.iptr(idx), _capi.dptr(elem),
+                                            fac.ctypes.data_as(_capi.P_DBL), _capi.dptr(sgn))
+        else:
+            if inst.nnz > np.iinfo(np.int32).max:
+                lib.HUserDataDestroy(C.byref(self.ud))
+                raise ValueError("instance exceeds the int32 CSC interface")
+            lib.HUserDataSetConeData(self.ud, 4, m, n, _capi.iptr(beg32), _capi.iptr(idx), _capi.dptr(elem))
+        if lib.HDSDPCreate(C.byref(self.h)):
+            lib.HUserDataDestroy(C.byref(self.ud))
+            raise RuntimeError("HDSDPCreate failed")
+        try:
+            if lib.HDSDPInit(self.h, m, 1):
+                raise RuntimeError("HDSDPInit failed")
+            lib.HDSDPSetVerbose(self.h, 1 if verbose else 0)
+            if lib.HDSDPSetCone(self.h, 0, self.ud):
+                raise RuntimeError("HDSDPSetCone failed")
+            lib.HDSDPSetDualObjective(self.h, _capi.dptr(b))
+            lib.HDSDPSetIntParamAny(self.h, INT_PARAM["psdp"], int(psdp))
+            if relopt is not None:
+                lib.HDSDPSetDblParam(self.h, DBL_PARAM["relopt"], float(relopt))
+            if absopt is not None:
+                lib.HDSDPSetDblParam(self.h, DBL_PARAM["absopt"], float(absopt))
+            if maxiter is not None:
+                lib.HDSDPSetIntParam(self.h, INT_PARAM["maxiter"], int(maxiter))
+            if timelimit is not None:
+                lib.HDSDPSetDblParam(self.h, DBL_PARAM["timelimit"], float(timelimit))
+            lib.HDSDPSetPrecision(self.h, int(precision))
+            lib.HDSDPSetProfiling(self.h, 1 if profile else 0)
+        except BaseException:
+            self.close()
+            raise
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def close(self):
+        if self.h:
+            self.lib.HDSDPDestroy(C.byref(self.h))
+            self.h = C.c_void_p()
+        if self.ud:
+            self.lib.HUserDataDestroy(C.byref(self.ud))
+            self.ud = C.c_void_p()
+
+    def optimize(self, gap_target: float = 1e-6) -> SolveResult:
+        lib, m = self.lib, self.m
+        rc = lib.HDSDPOptimize(self.h, 1)
         y = np.zeros(m)
         pobj, dobj = C.c_double(), C.c_double()
-        lib.HDSDPGetRowDual(solver, C.byref(pobj), C.byref(dobj), _capi.dptr(y))
+        lib.HDSDPGetRowDual(self.h, C.byref(pobj), C.byref(dobj), _capi.dptr(y))
         st, it = C.c_int(), C.c_int()
         mu, secs, secs_gap = C.c_double(), C.c_double(), C.c_double()
-        lib.HDSDPGetSummary(solver, gap_target, C.byref(st), C.byref(it), C.byref(mu),
+        lib.HDSDPGetSummary(self.h, gap_target, C.byref(st), C.byref(it), C.byref(mu),
                             C.byref(secs), C.byref(secs_gap))
-        nlog = lib.HDSDPGetIterLog(solver, 0, None)
+        nlog = lib.HDSDPGetIterLog(self.h, 0, None)
         log = np.zeros((max(nlog, 0), 9))
         if nlog > 0:
-            lib.HDSDPGetIterLog(solver, nlog, _capi.dptr(log))
+            lib.HDSDPGetIterLog(self.h, nlog, _capi.dptr(log))
         dimacs = np.zeros(6)
-        lib.HDSDPGetDimacsErrors(solver, _capi.dptr(dimacs))
+        lib.HDSDPGetDimacsErrors(self.h, _capi.dptr(dimacs))
         prof = None
-        if profile:
+        if self.profile:
             ns = len(_capi.PROF_SLOTS)
             ms, cnt = np.zeros(ns), np.zeros(ns)
-            lib.HDSDPGetProfile(solver, _capi.dptr(ms), _capi.dptr(cnt), ns)
+            lib.HDSDPGetProfile(self.h, _capi.dptr(ms), _capi.dptr(cnt), ns)
             prof = {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(_capi.PROF_SLOTS)}
-        res = SolveResult(int(rc), int(st.value), int(it.value), y, pobj.value, dobj.value, mu.value,
-                          secs.value, secs_gap.value, dimacs, log, prof)
-        if keep is not None:
-            keep.append((solver, ud, arrays))
-            solver = None
-        return res
-    finally:
-        if solver is not None:
-            lib.HDSDPDestroy(C.byref(solver))
-            lib.HUserDataDestroy(C.byref(ud))
+        return SolveResult(int(rc), int(st.value), int(it.value), y, pobj.value, dobj.value, mu.value,
+                           secs.value, secs_gap.value, dimacs, log, prof)
+
+    def setup(self) -> None:
+        """Pre-solve and device set-up without the interior-point iterations."""
+        if self.lib.HDSDPSetup(self.h):
+            raise RuntimeError("HDSDPSetup failed")
+
+    def barrier_point(self) -> tuple[float, np.ndarray]:
+        """(mu, y) of the last optimize, in the units of the data."""
+        mu, y = C.c_double(), np.zeros(self.m)
+        if self.lib.HDSDPGetBarrierPoint(self.h, C.byref(mu), _capi.dptr(y)):
+            raise RuntimeError("no barrier point (optimize first)")
+        return mu.value, y
+
+    def center(self, mu: float, y0: np.ndarray, maxiter: int = 60, tol: float = 1e-13):
+        """Newton-centre y0 at mu: returns (y_mu, Newton steps, relative residual)."""
+        y = np.ascontiguousarray(y0, dtype=np.float64).copy()
+        nit, res = C.c_int(), C.c_double()
+        rc = self.lib.HDSDPCenter(self.h, float(mu), _capi.dptr(y), int(maxiter), float(tol), C.byref(nit),
+                                  C.byref(res))
+        if rc:
+            raise RuntimeError(f"HDSDPCenter failed after {nit.value} steps (residual {res.value:.3e})")
+        return y, nit.value, res.value
+
+    def adjoint(self, mu: float, y: np.ndarray, dLdy: np.ndarray):
+        """(dL/db, dL/dC) of a loss of the centred y_mu from dL/dy."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        g = np.ascontiguousarray(dLdy, dtype=np.float64)
+        db = np.zeros(self.m)
+        dC = np.zeros((self.n, self.n), order="F")
+        if self.lib.HDSDPAdjoint(self.h, float(mu), _capi.dptr(y), _capi.dptr(g), _capi.dptr(db),
+                                 dC.ctypes.data_as(_capi.P_DBL)):
+            raise RuntimeError("HDSDPAdjoint failed")
+        return db, np.ascontiguousarray(dC)
+
+
+def run_solver(lib: C.CDLL, inst: SDPInstance, *, gap_target: float = 1e-6, **opts) -> SolveResult:
+    """Solve ``inst`` through the C API of ``lib`` (options as for ``Solver``)."""
+    with Solver(lib, inst, **opts) as sv:
+        return sv.optimize(gap_target)
 
 
 def solve(inst: SDPInstance, **kw) -> SolveResult:

@@ -146,6 +146,33 @@ void HDSDPSetVerbose( hdsdp *HSolver, int verbose );
 void HDSDPSetProfiling( hdsdp *HSolver, int enable );
 int HDSDPGetProfile( hdsdp *HSolver, double *ms, double *counts, int nSlots );
 
+/* Pre-solve and device set-up without the interior-point iterations (what
+ * HDSDPOptimize does first; HDSDPOptimize skips it when it has been done), so
+ * that HDSDPCenter can work on a problem that was never solved. */
+hdsdp_retcode HDSDPSetup( hdsdp *HSolver );
+
+/* The centred dual point and its adjoint.  For mu > 0, y_mu(b, C) maximises
+ * b^T y + mu log det(C - sum_i y_i A_i) (the dual box of the interior-point
+ * method is not part of it).  All values are in the units of the data given
+ * to the solver.  Callers: differentiation through the solve
+ * (hdsdp_amd.adjoint), which needs a smooth map of (b, C). */
+
+/* Barrier parameter and dual iterate of the last HDSDPOptimize. */
+hdsdp_retcode HDSDPGetBarrierPoint( hdsdp *HSolver, double *mu, double *y );
+
+/* Damped Newton from y (in/out, S(y) positive definite) to y_mu, until
+ * ||b - mu A(S^-1)|| <= tol (1 + ||b||) or maxIter steps (FAILED then).
+ * Leaves the dual factor and Schur matrix of the final y on the device. */
+hdsdp_retcode HDSDPCenter( hdsdp *HSolver, double mu, double *y, int maxIter, double tol,
+                           int *nIter, double *relRes );
+
+/* Gradients of a loss L(y_mu) from dLdy at the centred y: with
+ * M_ij = <A_i, S^-1 A_j S^-1>, M lambda = dLdy, dLdb = lambda / mu and
+ * dLdC = S^-1 (sum_i lambda_i A_i) S^-1 (n x n, full, column-major).  Reuses
+ * the factor and Schur matrix that HDSDPCenter left for this (mu, y). */
+hdsdp_retcode HDSDPAdjoint( hdsdp *HSolver, double mu, const double *y, const double *dLdy,
+                            double *dLdb, double *dLdC );
+
 #ifdef __cplusplus
 }
 #endif

@@ -143,10 +143,19 @@ hdsdp_retcode hdsdp_dev_get_dual( hdsdp_dev *dev, double *S );
 
 /* ---- adjoint (no counterpart in the reference) -------------------------- */
 
-/* Differentiates the barrier optimality conditions M(y) dy = ... at the final
- * mu with the last Schur matrix: solves M lambda = dLdy, then
- * dLdb = lambda / mu and dLdC = -S^-1 (sum_i lambda_i A_i) S^-1 (n x n full).
- * Callers: autograd through the solver (hdsdp_amd.adjoint). */
+/* Direct FP64 solve of M x = rhs with the current Schur matrix: FP64 Gram
+ * blocks (rebuilt if they hold bf16x3 products), FP64 Cholesky, two steps of
+ * iterative refinement.  FAILED if M is not positive definite.
+ * Callers: the centring Newton steps and the adjoint, which need solutions to
+ * working accuracy rather than to the interior-point CG tolerance. */
+hdsdp_retcode hdsdp_dev_kkt_solve_exact( hdsdp_dev *dev, const double *rhs, double *sol );
+
+/* Differentiates the barrier optimality conditions b = mu A(S^-1),
+ * S = C - sum_i y_i A_i, at the current dual factor and Schur matrix:
+ * solves M lambda = dLdy exactly, then dLdb = lambda / mu and
+ * dLdC = S^-1 (sum_i lambda_i A_i) S^-1 (n x n full).  In the form
+ * -mu S^-1 (sum_i l_i A_i) S^-1 this is l = -lambda / mu.
+ * Callers: HDSDPAdjoint (autograd through the solver, hdsdp_amd.adjoint). */
 hdsdp_retcode hdsdp_dev_adjoint( hdsdp_dev *dev, double mu, const double *dLdy,
                                  double *dLdb, double *dLdC );
 

@@ -750,6 +750,30 @@ hdsdp_retcode hdsdp_dev_kkt_solve( hdsdp_dev *d, const double *rhs, double *sol 
     return cg_solve(d, d->rhsbuf, sol);
 }
 
+/* Direct FP64 solve of M x = rhs: Cholesky of M, then two steps of iterative
+ * refinement on the residual with M. */
+hdsdp_retcode hdsdp_dev_kkt_solve_exact( hdsdp_dev *d, const double *rhs, double *sol ) {
+    const int m = d->m;
+    memcpy(d->Mchol, d->M, sizeof(double) * (size_t) m * m);
+    d->stats[4] += 1;
+    if ( ochol(m, d->Mchol) != 0 ) return HDSDP_RETCODE_FAILED;
+    double *r = (double *) calloc((size_t) 2 * m, sizeof(double));
+    if ( !r ) return HDSDP_RETCODE_MEMORY;
+    double *Mx = r + m;
+    memcpy(sol, rhs, sizeof(double) * m);
+    otrsv(m, d->Mchol, sol, 0);
+    otrsv(m, d->Mchol, sol, 1);
+    for ( int it = 0; it < 2; ++it ) {
+        osymv(m, 1.0, d->M, sol, Mx);
+        for ( int i = 0; i < m; ++i ) r[i] = rhs[i] - Mx[i];
+        otrsv(m, d->Mchol, r, 0);
+        otrsv(m, d->Mchol, r, 1);
+        for ( int i = 0; i < m; ++i ) sol[i] += r[i];
+    }
+    free(r);
+    return HDSDP_RETCODE_OK;
+}
+
 hdsdp_retcode hdsdp_dev_get_schur( hdsdp_dev *d, double *M ) {
     int m = d->m;
     for ( int j = 0; j < m; ++j )
@@ -825,7 +849,7 @@ hdsdp_retcode hdsdp_dev_get_dual( hdsdp_dev *d, double *S ) {
 hdsdp_retcode hdsdp_dev_adjoint( hdsdp_dev *d, double mu, const double *dLdy, double *dLdb, double *dLdC ) {
     int m = d->m, n = d->n;
     double *lam = (double *) calloc(m, sizeof(double));
-    hdsdp_retcode rc = hdsdp_dev_kkt_solve(d, dLdy, lam);
+    hdsdp_retcode rc = hdsdp_dev_kkt_solve_exact(d, dLdy, lam);
     if ( rc != HDSDP_RETCODE_OK ) { free(lam); return rc; }
     for ( int i = 0; i < m; ++i ) dLdb[i] = lam[i] / mu;
     oinverse(d, d->L[0], d->W);
@@ -834,7 +858,11 @@ hdsdp_retcode hdsdp_dev_adjoint( hdsdp_dev *d, double mu, const double *dLdy, do
     osymmetrize_lower(n, d->tmp2);
     ogemm(n, d->W, d->tmp2, d->tmp);
     ogemm(n, d->tmp, d->W, dLdC);
-    for ( size_t k = 0; k < (size_t) n * n; ++k ) dLdC[k] = -dLdC[k];
+    for ( int j = 0; j < n; ++j )
+        for ( int i = j + 1; i < n; ++i ) {
+            double v = 0.5 * (FE(dLdC, n, i, j) + FE(dLdC, n, j, i));
+            FE(dLdC, n, i, j) = FE(dLdC, n, j, i) = v;
+        }
     free(lam);
     return HDSDP_RETCODE_OK;
 }

@@ -258,3 +258,22 @@ def test_golden_fixtures_are_consistent():
         assert mg.instance_digest(make_inst()) == fix["digest"], name
         last = fix["log"][-1]
         assert abs(last[3] - fix["dobj"]) <= 1e-7 * max(1.0, abs(fix["dobj"])), name
+
+
+# ---- adjoint of the centred map (FD pins the formula and its sign) ----------
+
+ADJOINT_CASES = {
+    "maxcut_n40": lambda: I.maxcut(40, 0.2, 5, "unif"),
+    "dense_n12_m20": lambda: I.dense_random(12, 20, 3),
+}
+
+
+@pytest.mark.parametrize("name", list(ADJOINT_CASES))
+def test_adjoint_matches_central_fd(oracle, name):
+    """dL/db = lambda / mu and dL/dC = +S^-1 (sum lambda_i A_i) S^-1 with M lambda = dL/dy
+    reproduce central finite differences (h = 1e-6) of the centred dual map within 1e-5
+    relative (a flipped sign or a dropped 1/mu fails by orders of magnitude)."""
+    errs, mu, db, dC, y = dt.adjoint_fd_check(oracle, ADJOINT_CASES[name](), 4)
+    print(name, mu, errs)
+    assert max(errs) < 1e-5
+    assert np.array_equal(dC, dC.T)
