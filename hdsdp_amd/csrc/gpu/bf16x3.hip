@@ -69,8 +69,19 @@ __device__ inline void store16(__bf16 *hi, __bf16 *lo, const float (&v)[16]) {
 template <typename Tout, bool LOWER>
 __global__ __launch_bounds__(256) void gemm_bf16x3_tn_kernel(int M, int N, int K, double alpha, const float *X, int ldx,
                                                              long sX, const float *Y, int ldy, long sY, double beta,
-                                                             Tout *C, int ldc, long sC) {
-    const int i0 = blockIdx.x * XT, j0 = blockIdx.y * XT;
+                                                             Tout *C, int ldc, long sC, int lin) {
+    // lin: blockIdx.x runs over the lower tiles only (t = x (x + 1) / 2 + y, y <= x), so
+    // that consecutive workgroups, dealt to consecutive XCDs, share the triangle evenly
+    int bx = blockIdx.x, by = blockIdx.y;
+    if (lin) {
+        const int t = blockIdx.x;
+        int x = (int) ((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+        while ((x + 1) * (x + 2) / 2 <= t) ++x;
+        while (x * (x + 1) / 2 > t) --x;
+        bx = x;
+        by = t - x * (x + 1) / 2;
+    }
+    const int i0 = bx * XT, j0 = by * XT;
     if (LOWER && i0 + XT - 1 < j0) return;
     X += (long) blockIdx.z * sX;
     Y += (long) blockIdx.z * sY;
@@ -179,12 +190,15 @@ template <typename Tout>
 void gemm_bf16x3_tn(hipStream_t s, bool lower, int M, int N, int K, double alpha, const float *X, int ldx, long sX,
                     const float *Y, int ldy, long sY, double beta, Tout *C, int ldc, long sC, int batch) {
     if (M <= 0 || N <= 0 || batch <= 0) return;
-    dim3 grid((M + XT - 1) / XT, (N + XT - 1) / XT, batch);
+    const int mt = (M + XT - 1) / XT, nt = (N + XT - 1) / XT;
+    const int lin = (lower && mt == nt) ? 1 : 0;
+    dim3 grid(lin ? mt * (mt + 1) / 2 : mt, lin ? 1 : nt, batch);
     if (lower)
-        gemm_bf16x3_tn_kernel<Tout, true><<<grid, 256, 0, s>>>(M, N, K, alpha, X, ldx, sX, Y, ldy, sY, beta, C, ldc, sC);
+        gemm_bf16x3_tn_kernel<Tout, true><<<grid, 256, 0, s>>>(M, N, K, alpha, X, ldx, sX, Y, ldy, sY, beta, C, ldc, sC,
+                                                               lin);
     else
         gemm_bf16x3_tn_kernel<Tout, false><<<grid, 256, 0, s>>>(M, N, K, alpha, X, ldx, sX, Y, ldy, sY, beta, C, ldc,
-                                                                sC);
+                                                                sC, 0);
 }
 
 template void gemm_bf16x3_tn<float>(hipStream_t, bool, int, int, int, double, const float *, int, long, const float *,

@@ -46,12 +46,33 @@ __device__ inline void tri_krange(int tri, int i0, int j0, int tile, int K, int 
     if (tri == 2) kHi = (i0 + tile < K) ? i0 + tile : K;
 }
 
+// Output tile of this workgroup.  With lin != 0 (lower products with as many tile rows
+// as tile columns) blockIdx.x runs over the lower tiles only, t = x (x + 1) / 2 + y with
+// y <= x: consecutive workgroups go to consecutive XCDs, so every XCD gets the same
+// share of the triangle (a 2-D grid gives tile column x to XCD x mod 8, and the
+// columns near the diagonal end hold unequal work).
+__device__ inline void tile_xy(int lin, int &bx, int &by) {
+    if (!lin) {
+        bx = blockIdx.x;
+        by = blockIdx.y;
+        return;
+    }
+    const int t = blockIdx.x;
+    int x = (int) ((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((x + 1) * (x + 2) / 2 <= t) ++x;
+    while (x * (x + 1) / 2 > t) --x;
+    bx = x;
+    by = t - x * (x + 1) / 2;
+}
+
 template <typename T, bool TA, bool TB, bool LOWER>
 __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha, const T *A, int lda,
                                                    long sA, const T *B, int ldb, long sB, T beta,
-                                                   T *C, int ldc, long sC, int tri) {
+                                                   T *C, int ldc, long sC, int tri, int lin) {
     typedef typename Mfma<T>::acc_t acc_t;
-    const int i0 = blockIdx.x * TILE, j0 = blockIdx.y * TILE;
+    int bx, by;
+    tile_xy(lin, bx, by);
+    const int i0 = bx * TILE, j0 = by * TILE;
     if (LOWER && i0 + TILE - 1 < j0) return;
     int kLo, kHi;
     tri_krange(tri, i0, j0, TILE, K, kLo, kHi);
@@ -146,14 +167,80 @@ __device__ inline void big_load(const T *A, int lda, const T *B, int ldb, int M,
     }
 }
 
-// one k slab of the tile into registers; the bounds checks are dropped for interior slabs
-template <typename T, bool TA, bool TB>
-__device__ inline void big_fetch(const T *A, int lda, const T *B, int ldb, int M, int N, int K, int i0, int j0,
+// Interior slab with 16-byte loads (FP64; leading dimensions, strides and base
+// addresses even, checked by the launcher).  Each thread moves 8 elements of each
+// operand as 4 pairs:
+//   k-contiguous operand (A with TA, B without TB): row tid / 2, k = 8 (tid % 2) + 2u
+//   row-contiguous operand: k = tid / 16, rows 2 (tid % 16) + 32u, so that the 16
+//   lanes of one k read 256 consecutive bytes per instruction.
+typedef double dpair __attribute__((ext_vector_type(2)));
+
+template <bool TA, bool TB>
+__device__ inline void big_load_vec(const double *A, int lda, const double *B, int ldb, int i0, int j0, int k0,
+                                    int tid, double (&ra)[8], double (&rb)[8]) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        dpair va, vb;
+        if (TA) {
+            const int i = tid >> 1, k = (tid & 1) * 8 + 2 * u;
+            va = *reinterpret_cast<const dpair *>(A + (k0 + k) + (long) (i0 + i) * lda);
+        } else {
+            const int k = tid >> 4, i = 2 * (tid & 15) + 32 * u;
+            va = *reinterpret_cast<const dpair *>(A + (i0 + i) + (long) (k0 + k) * lda);
+        }
+        if (!TB) {
+            const int j = tid >> 1, k = (tid & 1) * 8 + 2 * u;
+            vb = *reinterpret_cast<const dpair *>(B + (k0 + k) + (long) (j0 + j) * ldb);
+        } else {
+            const int k = tid >> 4, j = 2 * (tid & 15) + 32 * u;
+            vb = *reinterpret_cast<const dpair *>(B + (j0 + j) + (long) (k0 + k) * ldb);
+        }
+        ra[2 * u] = va.x;
+        ra[2 * u + 1] = va.y;
+        rb[2 * u] = vb.x;
+        rb[2 * u + 1] = vb.y;
+    }
+}
+
+template <bool TA, bool TB>
+__device__ inline void big_store_vec(double (*As)[BT + BPAD], double (*Bs)[BT + BPAD], int tid, const double (&ra)[8],
+                                     const double (&rb)[8]) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        if (TA) {
+            const int i = tid >> 1, k = (tid & 1) * 8 + 2 * u;
+            As[k][i] = ra[2 * u];
+            As[k + 1][i] = ra[2 * u + 1];
+        } else {
+            const int k = tid >> 4, i = 2 * (tid & 15) + 32 * u;
+            *reinterpret_cast<dpair *>(&As[k][i]) = (dpair){ra[2 * u], ra[2 * u + 1]};
+        }
+        if (!TB) {
+            const int j = tid >> 1, k = (tid & 1) * 8 + 2 * u;
+            Bs[k][j] = rb[2 * u];
+            Bs[k + 1][j] = rb[2 * u + 1];
+        } else {
+            const int k = tid >> 4, j = 2 * (tid & 15) + 32 * u;
+            *reinterpret_cast<dpair *>(&Bs[k][j]) = (dpair){rb[2 * u], rb[2 * u + 1]};
+        }
+    }
+}
+
+// one k slab of the tile into registers (returns whether the vector layout was used);
+// the bounds checks are dropped for interior slabs
+template <typename T, bool TA, bool TB, bool VEC>
+__device__ inline bool big_fetch(const T *A, int lda, const T *B, int ldb, int M, int N, int K, int i0, int j0,
                                  int k0, int tid, T (&ra)[8], T (&rb)[8]) {
-    if (i0 + BT <= M && j0 + BT <= N && k0 + KSTEP <= K)
+    if (i0 + BT <= M && j0 + BT <= N && k0 + KSTEP <= K) {
+        if constexpr (VEC) {
+            big_load_vec<TA, TB>(A, lda, B, ldb, i0, j0, k0, tid, ra, rb);
+            return true;
+        }
         big_load<T, TA, TB, false>(A, lda, B, ldb, M, N, K, i0, j0, k0, tid, ra, rb);
-    else
+    } else {
         big_load<T, TA, TB, true>(A, lda, B, ldb, M, N, K, i0, j0, k0, tid, ra, rb);
+    }
+    return false;
 }
 
 template <typename T, bool TA, bool TB>
@@ -170,20 +257,22 @@ __device__ inline void big_store(T (*As)[BT + BPAD], T (*Bs)[BT + BPAD], int tid
     }
 }
 
-template <typename T, bool TA, bool TB, bool LOWER>
+template <typename T, bool TA, bool TB, bool LOWER, bool VEC>
 __global__ __launch_bounds__(256) void gemm_big_kernel(int M, int N, int K, T alpha, const T *A, int lda, long sA,
                                                        const T *B, int ldb, long sB, T beta, T *C, int ldc,
-                                                       long sC, int tri) {
+                                                       long sC, int tri, int lin) {
     typedef typename Mfma<T>::acc_t acc_t;
-    const int i0 = blockIdx.x * BT, j0 = blockIdx.y * BT;
+    int bx, by;
+    tile_xy(lin, bx, by);
+    const int i0 = bx * BT, j0 = by * BT;
     if (LOWER && i0 + BT - 1 < j0) return;
     int kLo, kHi;
     tri_krange(tri, i0, j0, BT, K, kLo, kHi);
     A += (long) blockIdx.z * sA;
     B += (long) blockIdx.z * sB;
     C += (long) blockIdx.z * sC;
-    __shared__ T As[2][KSTEP][BT + BPAD];
-    __shared__ T Bs[2][KSTEP][BT + BPAD];
+    __shared__ __attribute__((aligned(16))) T As[2][KSTEP][BT + BPAD];
+    __shared__ __attribute__((aligned(16))) T Bs[2][KSTEP][BT + BPAD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wi = (wave & 1) * 64, wj = (wave >> 1) * 64;
     const int lr = lane & 15, lk = lane >> 4;
@@ -193,14 +282,23 @@ __global__ __launch_bounds__(256) void gemm_big_kernel(int M, int N, int K, T al
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = (acc_t){0, 0, 0, 0};
     T ra[8], rb[8];
-    big_fetch<T, TA, TB>(A, lda, B, ldb, M, N, K, i0, j0, kLo, tid, ra, rb);
-    big_store<T, TA, TB>(As[0], Bs[0], tid, ra, rb);
+    auto stash = [&](int buf, bool vec) {
+        if constexpr (VEC) {
+            if (vec) {
+                big_store_vec<TA, TB>(As[buf], Bs[buf], tid, ra, rb);
+                return;
+            }
+        }
+        big_store<T, TA, TB>(As[buf], Bs[buf], tid, ra, rb);
+    };
+    bool vec = big_fetch<T, TA, TB, VEC>(A, lda, B, ldb, M, N, K, i0, j0, kLo, tid, ra, rb);
+    stash(0, vec);
     __syncthreads();
     const int nslab = (kHi - kLo + KSTEP - 1) / KSTEP;
     for (int sl = 0; sl < nslab; ++sl) {
         const int cur = sl & 1;
         const bool more = sl + 1 < nslab;
-        if (more) big_fetch<T, TA, TB>(A, lda, B, ldb, M, N, K, i0, j0, kLo + (sl + 1) * KSTEP, tid, ra, rb);
+        if (more) vec = big_fetch<T, TA, TB, VEC>(A, lda, B, ldb, M, N, K, i0, j0, kLo + (sl + 1) * KSTEP, tid, ra, rb);
 #pragma unroll
         for (int kk = 0; kk < KSTEP; kk += 4) {
             T af[4], bf[4];
@@ -214,7 +312,7 @@ __global__ __launch_bounds__(256) void gemm_big_kernel(int M, int N, int K, T al
 #pragma unroll
                 for (int b = 0; b < 4; ++b) acc[a][b] = Mfma<T>::op(af[a], bf[b], acc[a][b]);
         }
-        if (more) big_store<T, TA, TB>(As[cur ^ 1], Bs[cur ^ 1], tid, ra, rb);
+        if (more) stash(cur ^ 1, vec);
         __syncthreads();
     }
     // Epilogue: each wave stages its 64 x 64 quadrant through LDS (the k-slab buffers,
@@ -254,12 +352,22 @@ void gemm(hipStream_t s, bool transA, bool transB, bool lower, int M, int N, int
     // the 128-wide tiles pay off only when the k loop is long enough to hide their
     // prologue and epilogue (measured: rank-64 updates run faster on 64-wide tiles)
     if (bigTiles >= 192 && K >= 256) {
-        dim3 bgrid((M + BT - 1) / BT, (N + BT - 1) / BT, batch);
-#define HDK_GEMM_BIG(ta, tb, lo)                                                                          \
-        if (transA == ta && transB == tb && lower == lo) {                                                \
-            gemm_big_kernel<T, ta, tb, lo><<<bgrid, 256, 0, s>>>(M, N, K, alpha, A, lda, strideA, B, ldb, \
-                                                                 strideB, beta, C, ldc, strideC, tri);    \
-            return;                                                                                       \
+        const int mt = (M + BT - 1) / BT, nt = (N + BT - 1) / BT;
+        const int lin = (lower && mt == nt) ? 1 : 0;
+        dim3 bgrid(lin ? mt * (mt + 1) / 2 : mt, lin ? 1 : nt, batch);
+        // 16-byte loads need even leading dimensions, strides and addresses
+        const bool vec = sizeof(T) == 8 && lda % 2 == 0 && ldb % 2 == 0 && strideA % 2 == 0 && strideB % 2 == 0 &&
+                         ((uintptr_t) A % 16) == 0 && ((uintptr_t) B % 16) == 0;
+#define HDK_GEMM_BIG(ta, tb, lo)                                                                                \
+        if (transA == ta && transB == tb && lower == lo) {                                                      \
+            if (vec)                                                                                            \
+                gemm_big_kernel<T, ta, tb, lo, sizeof(T) == 8><<<bgrid, 256, 0, s>>>(                           \
+                    M, N, K, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc, strideC, tri, lin);         \
+            else                                                                                                \
+                gemm_big_kernel<T, ta, tb, lo, false><<<bgrid, 256, 0, s>>>(M, N, K, alpha, A, lda, strideA, B, \
+                                                                            ldb, strideB, beta, C, ldc,         \
+                                                                            strideC, tri, lin);                 \
+            return;                                                                                             \
         }
         HDK_GEMM_BIG(false, false, false)
         HDK_GEMM_BIG(false, true, false)
@@ -271,11 +379,13 @@ void gemm(hipStream_t s, bool transA, bool transB, bool lower, int M, int N, int
         HDK_GEMM_BIG(true, true, true)
 #undef HDK_GEMM_BIG
     }
-    dim3 grid((M + TILE - 1) / TILE, (N + TILE - 1) / TILE, batch);
+    const int mt = (M + TILE - 1) / TILE, nt = (N + TILE - 1) / TILE;
+    const int lin = (lower && mt == nt) ? 1 : 0;
+    dim3 grid(lin ? mt * (mt + 1) / 2 : mt, lin ? 1 : nt, batch);
 #define HDK_GEMM_CASE(ta, tb, lo)                                                                    \
     if (transA == ta && transB == tb && lower == lo) {                                               \
         gemm_kernel<T, ta, tb, lo><<<grid, 256, 0, s>>>(M, N, K, alpha, A, lda, strideA, B, ldb, strideB, \
-                                                        beta, C, ldc, strideC, tri);                \
+                                                        beta, C, ldc, strideC, tri, lin);           \
         return;                                                                                      \
     }
     HDK_GEMM_CASE(false, false, false)
@@ -293,6 +403,57 @@ template void gemm<double>(hipStream_t, bool, bool, bool, int, int, int, double,
                            const double *, int, long, double, double *, int, long, int, int);
 template void gemm<float>(hipStream_t, bool, bool, bool, int, int, int, float, const float *, int, long,
                           const float *, int, long, float, float *, int, long, int, int);
+
+// ---- split-K ---------------------------------------------------------------------
+
+// C(i, j) = alpha sum_p P_p(i, j) + beta C(i, j) over the P partial products (ld M),
+// i >= j only when lower.  Partials are summed in index order.
+template <typename T>
+__global__ void splitk_reduce_kernel(int M, int N, int P, T alpha, const T *W, T beta, T *C, int ldc, int lower) {
+    const long total = (long) M * N, mn = total;
+    for (long e = blockIdx.x * (long) blockDim.x + threadIdx.x; e < total; e += (long) gridDim.x * blockDim.x) {
+        const int i = (int) (e % M), j = (int) (e / M);
+        if (lower && i < j) continue;
+        T acc = W[e];
+        for (int p = 1; p < P; ++p) acc += W[e + p * mn];
+        T *c = C + i + (long) j * ldc;
+        *c = (beta == (T) 0) ? alpha * acc : alpha * acc + beta * *c;
+    }
+}
+
+int splitk_parts(int M, int N, int K, bool lower) {
+    const long tiles = std::max<long>(1, (long) ((M + BT - 1) / BT) * ((N + BT - 1) / BT) / (lower ? 2 : 1));
+    if (tiles >= 192) return 1;
+    // enough 128-wide tiles for two rounds over the CUs, at least 1024 of k per part
+    long P = std::min<long>(16, (512 + tiles - 1) / tiles);
+    P = std::min<long>(P, K / 1024);
+    return (int) std::max<long>(P, 1);
+}
+
+template <typename T>
+void gemm_splitk(hipStream_t s, bool transA, bool transB, bool lower, int M, int N, int K, T alpha, const T *A,
+                 int lda, const T *B, int ldb, T beta, T *C, int ldc, T *work, size_t workLen) {
+    if (M <= 0 || N <= 0) return;
+    const size_t mn = (size_t) M * N;
+    long P = splitk_parts(M, N, K, lower);
+    if (work) P = std::min<long>(P, (long) (workLen / mn));
+    if (!work || P <= 1) {
+        gemm<T>(s, transA, transB, lower, M, N, K, alpha, A, lda, 0, B, ldb, 0, beta, C, ldc, 0, 1, 0);
+        return;
+    }
+    const int kc = (int) (((K + P - 1) / P + 15) / 16 * 16);
+    const int full = K / kc, rem = K - full * kc;
+    const long sA = transA ? kc : (long) kc * lda, sB = transB ? (long) kc * ldb : kc;
+    gemm<T>(s, transA, transB, lower, M, N, kc, (T) 1, A, lda, sA, B, ldb, sB, (T) 0, work, M, (long) mn, full, 0);
+    if (rem > 0)
+        gemm<T>(s, transA, transB, lower, M, N, rem, (T) 1, A + full * sA, lda, 0, B + full * sB, ldb, 0, (T) 0,
+                work + full * mn, M, 0, 1, 0);
+    splitk_reduce_kernel<T><<<4096, 256, 0, s>>>(M, N, full + (rem > 0 ? 1 : 0), alpha, work, beta, C, ldc,
+                                                 lower ? 1 : 0);
+}
+
+template void gemm_splitk<double>(hipStream_t, bool, bool, bool, int, int, int, double, const double *, int,
+                                  const double *, int, double, double *, int, double *, size_t);
 
 // ---- Cholesky ----------------------------------------------------------------
 

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 
@@ -32,6 +33,16 @@ template <typename T>
 void gemm(hipStream_t s, bool transA, bool transB, bool lower, int M, int N, int K, T alpha,
           const T *A, int lda, long strideA, const T *B, int ldb, long strideB, T beta, T *C,
           int ldc, long strideC, int batch = 1, int tri = 0);
+
+// gemm with the k range split into parts whose partial products go to work (P x M x N)
+// and are summed in a fixed order, for products with few output tiles and long k
+// (128-wide tiles fill the chip only with enough of them).  Falls back to gemm when
+// work is null or too small for two parts.
+// number of k parts gemm_splitk uses for this shape (1: no split)
+int splitk_parts(int M, int N, int K, bool lower);
+template <typename T>
+void gemm_splitk(hipStream_t s, bool transA, bool transB, bool lower, int M, int N, int K, T alpha, const T *A,
+                 int lda, const T *B, int ldb, T beta, T *C, int ldc, T *work, size_t workLen);
 
 // C = alpha * X^T Y + beta * C in FP32-equivalent precision on the bf16 matrix cores
 // (three-term bf16 split products, FP32 accumulation flushed into FP64 every 1024 k).

@@ -54,6 +54,8 @@ struct hdsdp_dev {
     int schurMixed, mGramPending;
     double *Wm, *Lim, *mDiag0, *mDiagCur;
     double *xCoef;  // m + 1 coefficients of the matrix-free Schur product
+    double *splitW;  // k-split partials of the rank-nR1 buffer update
+    size_t splitLen;
 
     double *S[3], *L[2], *Li[2], *LiT[2];
     long facVer[2], liVer[2];
@@ -379,7 +381,7 @@ void symv_M(hdsdp_dev *d, const double *x, double *y) {
 void schur_apply_exact(hdsdp_dev *d, const double *x, double *y) {
     const int m = d->m, n = d->n;
     hdk::shift_coef(d->st, m, x, d->xCoef);
-    hdk::assemble_buffer(d->st, d->c, d->xCoef, 0.0, d->T1, 0.0, d->work);
+    hdk::assemble_buffer(d->st, d->c, d->xCoef, 0.0, d->T1, 0.0, d->work, d->splitW, d->splitLen);
     hdk::mirror_lower<double>(d->st, n, d->T1, n);
     hdk::gemm<double>(d->st, false, false, false, n, n, n, 1.0, d->Wm, n, 0, d->T1, n, 0, 0.0, d->T2, n, 0, 1);
     hdk::gemm<double>(d->st, false, false, false, n, n, n, 1.0, d->T2, n, 0, d->Wm, n, 0, 0.0, d->T1, n, 0, 1);
@@ -496,9 +498,12 @@ void dense_gram(hdsdp_dev *d, const double *Li) {
     for (int b0 = 0; b0 < r; b0 += d->gramBatch) {
         const int cnt = std::min(d->gramBatch, r - b0);
         hdk::unpack_dense(d->st, d->c, b0, cnt, d->Dbuf);
-        hdk::gemm<double>(d->st, false, false, false, n, n, n, 1.0, Li, n, 0, d->Dbuf, n, nn, 0.0, d->Ubuf, n, nn, cnt);
-        hdk::gemm<double>(d->st, false, true, false, n, n, n, 1.0, d->Ubuf, n, nn, Li, n, 0, 0.0,
-                          d->Tall + (size_t) b0 * nn, n, nn, cnt);
+        // U = L^-1 A_p, then T_p = U L^-T = L^-1 U^T (T_p symmetric): both products have
+        // the lower-triangular L^-1 on the left, whose zero k ranges are skipped
+        hdk::gemm<double>(d->st, false, false, false, n, n, n, 1.0, Li, n, 0, d->Dbuf, n, nn, 0.0, d->Ubuf, n, nn, cnt,
+                          2);
+        hdk::gemm<double>(d->st, false, true, false, n, n, n, 1.0, Li, n, 0, d->Ubuf, n, nn, 0.0,
+                          d->Tall + (size_t) b0 * nn, n, nn, cnt, 2);
     }
     const long kc = d->gramChunk;
     const int full = (int) (nn / kc);
@@ -834,6 +839,15 @@ hdsdp_retcode hdsdp_dev_load(hdsdp_dev *d, const hdsdp_coeff_view *cv) {
     DCALL(up_d(&c.smVal, smVal));
     size_t workSize = std::max((size_t) n * n, (size_t) n * std::max(c.nR1, 1));
     DCALL(dalloc(d, &d->work, workSize));
+    d->splitW = nullptr;
+    d->splitLen = 0;
+    if (c.nR1 > 0) {
+        const int parts = hdk::splitk_parts(n, n, c.nR1, true);
+        if (parts > 1) {
+            d->splitLen = (size_t) parts * n * n;
+            DCALL(dalloc(d, &d->splitW, d->splitLen));
+        }
+    }
     d->H1 = d->Kr = d->Tall = d->Dbuf = d->Ubuf = d->Gpart = nullptr;
     if (c.nR1 > 0 || c.cDSR1) DCALL(dalloc(d, &d->H1, (size_t) n * (c.nR1 + 1)));
     if (d->r1Gram && !d->r1Identity) DCALL(dalloc(d, &d->Kr, (size_t) c.nR1 * c.nR1));
@@ -867,10 +881,18 @@ hdsdp_retcode hdsdp_dev_load(hdsdp_dev *d, const hdsdp_coeff_view *cv) {
         const size_t nn = (size_t) n * n;
         const int r = d->nDn;
         d->gramBatch = (int) std::max<size_t>(1, std::min<size_t>(r, (size_t) (512u << 20) / (nn * sizeof(double))));
-        long tiles = (long) ((r + 63) / 64) * ((r + 63) / 64 + 1) / 2;
-        d->gramParts = (int) std::min<long>(16, std::max<long>(1, 2048 / tiles));
-        long K = (long) nn;
-        d->gramChunk = std::max<long>(256, ((K + d->gramParts - 1) / d->gramParts + 15) / 16 * 16);
+        // split-K parts of the Gram product: enough 128-wide lower tiles for ~8 rounds over
+        // the CUs, and a part count that divides n^2 where one exists, so that every part
+        // goes into one batched launch (a remainder part runs alone, on too few tiles)
+        long tiles = (long) ((r + 127) / 128) * ((r + 127) / 128 + 1) / 2;
+        long want = std::min<long>(16, std::max<long>(1, (2048 + tiles - 1) / tiles));
+        long K = (long) nn, parts = want;
+        for (long p = want; p <= 2 * want; ++p)
+            if (K % p == 0 && K / p >= 256) {
+                parts = p;
+                break;
+            }
+        d->gramChunk = (K % parts == 0) ? K / parts : std::max<long>(256, (K + parts - 1) / parts);
         d->gramParts = (int) ((K + d->gramChunk - 1) / d->gramChunk);
         if (mixedGram) {
             DCALL(dalloc(d, &d->Tf, nn * r));
@@ -994,7 +1016,7 @@ hdsdp_retcode hdsdp_dev_update_buffer(hdsdp_dev *d, double cCoef, double aScal, 
     HIP_OK(hipMemcpyAsync(d->coef_d, d->hcoef, sizeof(double) * (m + 1), hipMemcpyHostToDevice, d->st));
     d->stagingBusy = true;
     prof_begin(d, HDSDP_PROF_BUFFER);
-    hdk::assemble_buffer(d->st, d->c, d->coef_d, cCoef, d->S[which], dEyeCoef, d->work);
+    hdk::assemble_buffer(d->st, d->c, d->coef_d, cCoef, d->S[which], dEyeCoef, d->work, d->splitW, d->splitLen);
     DCALL(check_launch());
     return prof_end(d, HDSDP_PROF_BUFFER);
 }

@@ -40,8 +40,10 @@ struct DevCoeffs {
 
 // target = sum_k coef[k] A_k (coef[0] multiplies C) + eye I on the lower triangle.
 // cCoef is coef[0] on the host (needed for a dense rank-one C).  work: n * nR1.
+// splitWork (splitLen elements, may be null) holds the k-split partial products of the
+// rank-nR1 update F diag(w) F^T (splitk_parts(n, n, nR1, true) x n^2 used).
 void assemble_buffer(hipStream_t s, const DevCoeffs &c, const double *coef, double cCoef, double *target,
-                     double eye, double *work);
+                     double eye, double *work, double *splitWork = nullptr, size_t splitLen = 0);
 void axpy_buffer(hipStream_t s, long nn, double step, const double *src, const double *dS, double *dst);
 
 // out1[k - k0] = <A_k, B1>, out2[k - k0] = s2 <A_k, B2> for k in [k0, k0 + cnt); dense

@@ -93,7 +93,7 @@ __global__ void dsr1_weights_kernel(int nR1, const int *rCon, const double *csig
 }
 
 void assemble_buffer(hipStream_t s, const DevCoeffs &c, const double *coef, double cCoef, double *target,
-                     double eye, double *work) {
+                     double eye, double *work, double *splitWork, size_t splitLen) {
     const int n = c.n;
     dim3 g((n + 255) / 256, n);
     assemble_dense_kernel<<<g, 256, 0, s>>>(n, c.nDense, c.denseCon, c.dVal, coef, target);
@@ -103,7 +103,8 @@ void assemble_buffer(hipStream_t s, const DevCoeffs &c, const double *coef, doub
         // target += F diag(coef * sign) F^T (lower) as one GEMM
         dsr1_weights_kernel<<<(c.nR1 + 255) / 256, 256, 0, s>>>(c.nR1, c.rCon, c.csign, coef, c.r1w);
         scale_cols_kernel<<<4096, 256, 0, s>>>(n, c.nR1, c.rF, c.r1w, work);
-        gemm<double>(s, false, true, true, n, n, c.nR1, 1.0, work, n, 0, c.rF, n, 0, 1.0, target, n, 0, 1);
+        gemm_splitk<double>(s, false, true, true, n, n, c.nR1, 1.0, work, n, c.rF, n, 1.0, target, n, splitWork,
+                            splitLen);
     }
     if (c.cDSR1 && cCoef != 0.0) {
         // C = sign f f^T enters last among the coefficients, as a rank-one update

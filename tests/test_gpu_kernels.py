@@ -177,3 +177,25 @@ def test_schur_mixed_precision_gram(gpu_lib, name, monkeypatch):
     Mref = np.array([[np.sum(Ai * (W @ Aj @ W)) for Aj in A] for Ai in A])
     assert rel(M, Mref) < 1e-5
     assert rel(a, [np.sum(Ai * W) for Ai in A]) < 1e-10
+
+
+@pytest.mark.parametrize("n,m", [(260, 300), (130, 700)])
+def test_dense_schur_large_tiles(gpu_lib, n, m):
+    """Dense Schur matrices large enough for the 128-wide product kernels: the
+    triangular T = L^-1 A L^-T products (16-byte loads, skipped zero k ranges, edge
+    tiles) at n = 260, and the split-K Gram over lower tiles only (XCD-balanced
+    launch) at m = 700.  Against the closed form within 1e-9 relative."""
+    rng = np.random.default_rng(9)
+    coeffs = dt.rand_coeffs(rng, n, m, [dt.DENSE])
+    y, eye = dt.interior_point(coeffs, rng)
+    g = dt.Dev(gpu_lib, coeffs)
+    g.update(1.0, -1.0, y, eye, 0)
+    assert g.factorize(0) == 1
+    a, b, t = g.schur(0, 0.0)
+    M = g.get_schur()
+    W = np.linalg.inv(dt.dense_buffer(coeffs, 1.0, -1.0, y, eye))
+    L = np.linalg.cholesky(W)
+    T = np.stack([(L.T @ c.dense() @ L).ravel() for c in coeffs[1:]])
+    Mref = T @ T.T
+    assert rel(M, Mref) < 1e-9
+    assert rel(a, [np.sum(c.dense() * W) for c in coeffs[1:]]) < 1e-9
