@@ -1,11 +1,15 @@
 """Adjoint of the centred dual map on the GPU.
 
 The differentiated map is y_mu(b, C), the maximiser of b^T y + mu log det(C - A^T y)
-at the final barrier parameter of a 1e-8-gap solve; HDSDPAdjoint solves
+at the final barrier parameter of a solve; HDSDPAdjoint solves
 M lambda = dL/dy with the Schur matrix of the centred point and returns
 dL/db = lambda / mu, dL/dC = S^-1 (sum lambda_i A_i) S^-1.  Checked against
   * central finite differences (h = 1e-6) of centred GPU solves of the perturbed
-    data along seeded random directions, within 1e-5 relative;
+    data along seeded random directions, within 1e-5 relative, on solves to a
+    1e-7 gap: at the final mu S is near-singular (cond ~ 1/mu) and the centred
+    point carries an FP64 rounding floor of about eps cond(S), which the FD
+    quotient divides by 2h; at a 1e-8 gap (mu ~ 3e-10) that floor alone reaches
+    ~1e-5 on the dense instance (measured 2e-7 .. 1.1e-5 over directions);
   * the CPU oracle's adjoint at the same (mu, y), which pins the device kernels
     (Schur matrix, exact solve, the W B W products) against a plain C restatement;
   * the torch.autograd.Function wrapper, whose gradients must be those of
@@ -33,7 +37,7 @@ FD_CASES = {
 @pytest.mark.parametrize("name", list(FD_CASES))
 def test_adjoint_matches_central_fd(gpu_lib, name):
     make, ndir = FD_CASES[name]
-    errs, mu, db, dC, y = dt.adjoint_fd_check(gpu_lib, make(), ndir)
+    errs, mu, db, dC, y = dt.adjoint_fd_check(gpu_lib, make(), ndir, relopt=1e-7)
     print(name, "mu", mu, "rel errs", errs, flush=True)
     assert max(errs) < 1e-5
     assert np.array_equal(dC, dC.T)
