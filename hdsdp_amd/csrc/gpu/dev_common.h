@@ -44,12 +44,13 @@ template <typename T>
 void gemm_splitk(hipStream_t s, bool transA, bool transB, bool lower, int M, int N, int K, T alpha, const T *A,
                  int lda, const T *B, int ldb, T beta, T *C, int ldc, T *work, size_t workLen);
 
-// C = alpha * X^T Y + beta * C in FP32-equivalent precision on the bf16 matrix cores
-// (three-term bf16 split products, FP32 accumulation flushed into FP64 every 1024 k).
+// C = alpha * X^T Y + beta * C from bf16 split products on the matrix cores: each FP32
+// operand split into `terms` bf16 terms (2: three products, ~16 bits; 3: six products,
+// ~24 bits, FP32-equivalent), FP32 accumulation flushed into FP64 every 256 k.
 // X: K x M (ldx), Y: K x N (ldy), column-major FP32; lower restricts C to i >= j.
 template <typename Tout>
-void gemm_bf16x3_tn(hipStream_t s, bool lower, int M, int N, int K, double alpha, const float *X, int ldx, long sX,
-                    const float *Y, int ldy, long sY, double beta, Tout *C, int ldc, long sC, int batch = 1);
+void gemm_bf16s_tn(hipStream_t s, int terms, bool lower, int M, int N, int K, double alpha, const float *X, int ldx,
+                   long sX, const float *Y, int ldy, long sY, double beta, Tout *C, int ldc, long sC, int batch = 1);
 
 // A second stream and a ring of events for look-ahead factorisations.
 constexpr int LA_EVENTS = 16;

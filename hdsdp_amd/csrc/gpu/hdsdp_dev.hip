@@ -89,6 +89,11 @@ struct hdsdp_dev {
     double profMs[HDSDP_PROF_SLOTS], profCnt[HDSDP_PROF_SLOTS];
 };
 
+// bf16 terms per FP32 operand in the mixed-precision Gram products: three (six MFMA
+// products, ~24 bits) keeps the Gram blocks FP32-accurate, so that M stays positive
+// definite to a condition number near 1e7 before an FP64 rebuild is needed.
+constexpr int BF16_TERMS = 3;
+
 namespace {
 
 template <typename T>
@@ -331,9 +336,7 @@ hdsdp_retcode build_chol32(hdsdp_dev *d, int *ok) {
 
 hdsdp_retcode build_precond(hdsdp_dev *d) {
     const int m = d->m;
-    // bf16x3 Gram blocks: the CG runs on the matrix-free FP64 operator, where each
-    // product costs a Schur-sized pass, so it starts with the factor preconditioner
-    if (d->useJacobi && !d->mGramPending) {
+    if (d->useJacobi) {
         hdk::get_diag(d->st, m, d->M, m, d->jac);
         d->precKind = PREC_JACOBI;
         return check_launch();
@@ -376,7 +379,7 @@ void symv_M(hdsdp_dev *d, const double *x, double *y) {
 
 // y = M x in FP64 without M: (M x)_i = <A_i, W (sum_j x_j A_j) W> for the S^-1 = W of
 // the build, plus the diagonal terms added to M since (diag(M) - its built diagonal).
-// Used while the Gram blocks of M hold bf16x3 products: it makes the residuals of the
+// Used while the Gram blocks of M hold bf16-split products: it makes the residuals of the
 // refinement those of the exact FP64 system.
 void schur_apply_exact(hdsdp_dev *d, const double *x, double *y) {
     const int m = d->m, n = d->n;
@@ -394,16 +397,19 @@ void schur_apply_exact(hdsdp_dev *d, const double *x, double *y) {
 // Returns 0 on success, 1 if the solution is left unwritten (tiny rhs), -1 on failure.
 int cg_solve(hdsdp_dev *d) {
     const int m = d->m;
-    // While M holds bf16x3 Gram blocks the CG runs on the exact FP64 operator (applied
+    // While M holds bf16-split Gram blocks (FP32-accurate) the control flow is that of
+    // the FP64 path: Jacobi-preconditioned CG on that M while the reference would use
+    // Jacobi (its solves stop at a tolerance far above the error of M), and once it has
+    // switched to the factor preconditioner, CG on the exact FP64 operator (applied
     // matrix-free) with the Cholesky factor of that M as preconditioner, to 1e-4 tol
     // (near the accuracy of an FP64 factor solve); a solve that cannot get below tol
     // fails, and M is then rebuilt in FP64.
-    const bool exactOp = d->mGramPending != 0;
+    d->useJacobi = d->paramUseJacobi;
+    const bool exactOp = d->mGramPending != 0 && !d->useJacobi;
     auto op = [&](const double *x, double *y) {
         if (exactOp) schur_apply_exact(d, x, y);
         else symv_M(d, x, y);
     };
-    d->useJacobi = d->paramUseJacobi && !exactOp;
     const int restart = std::max(d->nRestart, 20);
     hdk::cg_start(d->st, m, d->crhs, d->cr, d->cx, d->sc_d);
     if (read_scalars(d) != HDSDP_RETCODE_OK) return -1;
@@ -561,8 +567,8 @@ void dense_gram_mixed(hdsdp_dev *d) {
     for (int b0 = 0; b0 < r; b0 += d->gramBatch) {
         const int cnt = std::min(d->gramBatch, r - b0);
         hdk::unpack_dense_f(d->st, d->c, b0, cnt, d->Dbuff);
-        hdk::gemm_bf16x3_tn<float>(d->st, false, n, n, n, 1.0, d->Dbuff, n, nn, d->LiTf, n, 0, 0.0, d->Vf, n, nn, cnt);
-        hdk::gemm_bf16x3_tn<float>(d->st, false, n, n, n, 1.0, d->LiTf, n, 0, d->Vf, n, nn, 0.0,
+        hdk::gemm_bf16s_tn<float>(d->st, BF16_TERMS, false, n, n, n, 1.0, d->Dbuff, n, nn, d->LiTf, n, 0, 0.0, d->Vf, n, nn, cnt);
+        hdk::gemm_bf16s_tn<float>(d->st, BF16_TERMS, false, n, n, n, 1.0, d->LiTf, n, 0, d->Vf, n, nn, 0.0,
                                    d->Tf + (size_t) b0 * nn, n, nn, cnt);
     }
     const long kc = d->gramChunk;
@@ -570,10 +576,10 @@ void dense_gram_mixed(hdsdp_dev *d) {
     const long rem = nn - (long) full * kc;
     const long rr = (long) r * r;
     if (full > 0)
-        hdk::gemm_bf16x3_tn<double>(d->st, true, r, r, (int) kc, 1.0, d->Tf, (int) nn, kc, d->Tf, (int) nn, kc, 0.0,
+        hdk::gemm_bf16s_tn<double>(d->st, BF16_TERMS, true, r, r, (int) kc, 1.0, d->Tf, (int) nn, kc, d->Tf, (int) nn, kc, 0.0,
                                     d->Gpart, r, rr, full);
     if (rem > 0)
-        hdk::gemm_bf16x3_tn<double>(d->st, true, r, r, (int) rem, 1.0, d->Tf + (long) full * kc, (int) nn, 0,
+        hdk::gemm_bf16s_tn<double>(d->st, BF16_TERMS, true, r, r, (int) rem, 1.0, d->Tf + (long) full * kc, (int) nn, 0,
                                     d->Tf + (long) full * kc, (int) nn, 0, 0.0, d->Gpart + (long) full * rr, r, 0, 1);
     hdk::gram_reduce(d->st, r, full + (rem > 0 ? 1 : 0), d->Gpart, r, rr, d->c.denseCon, d->M, m);
 }
@@ -582,10 +588,10 @@ void dense_gram_mixed(hdsdp_dev *d) {
 void r1_gram_mixed(hdsdp_dev *d) {
     const int n = d->n, r = d->c.nR1, m = d->m;
     hdk::d2f(d->st, (long) n * n, d->LiT[0], d->LiTf);
-    hdk::gemm_bf16x3_tn<float>(d->st, false, n, r, n, 1.0, d->LiTf, n, 0, d->Ff, n, 0, 0.0, d->Gf, n, 0, 1);
+    hdk::gemm_bf16s_tn<float>(d->st, BF16_TERMS, false, n, r, n, 1.0, d->LiTf, n, 0, d->Ff, n, 0, 0.0, d->Gf, n, 0, 1);
     double *K = d->r1Identity ? d->M : d->Kr;
     const long ldk = d->r1Identity ? m : r;
-    hdk::gemm_bf16x3_tn<double>(d->st, true, r, r, n, 1.0, d->Gf, n, 0, d->Gf, n, 0, 0.0, K, (int) ldk, 0, 1);
+    hdk::gemm_bf16s_tn<double>(d->st, BF16_TERMS, true, r, r, n, 1.0, d->Gf, n, 0, d->Gf, n, 0, 0.0, K, (int) ldk, 0, 1);
     hdk::r1_square(d->st, d->c, K, ldk, d->M, m);
 }
 
@@ -853,14 +859,14 @@ hdsdp_retcode hdsdp_dev_load(hdsdp_dev *d, const hdsdp_coeff_view *cv) {
     if (d->r1Gram && !d->r1Identity) DCALL(dalloc(d, &d->Kr, (size_t) c.nR1 * c.nR1));
     d->LiTf = d->Ff = d->Gf = d->Tf = d->Dbuff = d->Vf = nullptr;
     d->Wm = d->Lim = d->mDiag0 = d->mDiagCur = d->xCoef = nullptr;
-    // bf16x3 Gram blocks need the matrix-free FP64 product in the refinement.  For dense
+    // bf16-split Gram blocks need the matrix-free FP64 product in the refinement.  For dense
     // constraints it costs two passes over the packed coefficients and two n^3 products,
     // far below the Gram build; for rank-one constraints it costs about 4 n^2 r + 4 n^3
     // flops against n r^2 for the FP64 Gram, so they use it only when r >= 32 n.
-    // HDSDP_GRAM_BF16X3=1 (or 0) forces the choice where a Gram path exists.
+    // HDSDP_GRAM_BF16=1 (or 0) forces the choice where a Gram path exists.
     bool mixedGram = d->precision == HDSDP_PRECISION_MIXED &&
                      (d->dnGram || (d->r1Gram && (long) c.nR1 >= 32L * n));
-    if (const char *force = getenv("HDSDP_GRAM_BF16X3"))
+    if (const char *force = getenv("HDSDP_GRAM_BF16"))
         mixedGram = d->precision == HDSDP_PRECISION_MIXED && (d->r1Gram || d->dnGram) && atoi(force) != 0;
     d->schurMixed = mixedGram ? 1 : 0;
     d->mGramPending = 0;

@@ -116,8 +116,9 @@ void HDSDPSetIntParamAny( hdsdp *HSolver, int intParam, int intParamVal );
 void HDSDPSetDblParamAny( hdsdp *HSolver, int dblParam, double dblParamVal );
 
 /* Arithmetic of the Schur complement path: 0 = FP64 throughout (default);
- * 1 = mixed (BF16x3 split products with FP32 accumulation for dense and rank-one
- * Schur assembly, FP32 Cholesky of M as preconditioner for FP64 CG). */
+ * 1 = mixed (dense and rank-one Schur blocks from bf16 split products with FP32
+ * accumulation, FP32-equivalent; FP32 Cholesky of M as preconditioner of an FP64
+ * CG on the exact Schur operator; FP64 rebuild of M when that system fails). */
 #define HDSDP_PRECISION_FP64  0
 #define HDSDP_PRECISION_MIXED 1
 void HDSDPSetPrecision( hdsdp *HSolver, int precisionMode );

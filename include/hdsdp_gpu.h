@@ -144,7 +144,7 @@ hdsdp_retcode hdsdp_dev_get_dual( hdsdp_dev *dev, double *S );
 /* ---- adjoint (no counterpart in the reference) -------------------------- */
 
 /* Direct FP64 solve of M x = rhs with the current Schur matrix: FP64 Gram
- * blocks (rebuilt if they hold bf16x3 products), FP64 Cholesky, two steps of
+ * blocks (rebuilt if they hold bf16 split products), FP64 Cholesky, two steps of
  * iterative refinement.  FAILED if M is not positive definite.
  * Callers: the centring Newton steps and the adjoint, which need solutions to
  * working accuracy rather than to the interior-point CG tolerance. */

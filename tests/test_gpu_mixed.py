@@ -7,9 +7,9 @@ Two kinds of instance:
     Cholesky); the CG stops on FP64 residuals at the reference's tolerances,
     so the iterates agree to the solve tolerance: same status and iteration
     count;
-  * dense constraints (and rank-one ones when forced onto the bf16x3 Gram
-    path): the Gram blocks of M come from bf16x3 products (FP32-equivalent,
-    FP64 accumulation) and each Newton system is refined against the exact
+  * dense constraints (and rank-one ones when forced onto the bf16 Gram
+    path): the Gram blocks of M come from bf16 split products (three terms per
+    operand, FP32-equivalent, FP64 accumulation) and each Newton system is refined against the exact
     FP64 Schur operator applied matrix-free, so the directions carry FP64
     accuracy; when such an M cannot be factored or solved, or the refinement
     stalls, the library rebuilds M in FP64 and stays there.  The trajectory
@@ -20,10 +20,10 @@ y* within 1e-7, except on two rank-one instances, bound 1e-6:
     FP64 GPU run and the reference HDSDP differ by 2.2e-7 on it, and its
     trajectory is ulp-sensitive (a last-bit change in the factorisation moves
     the mixed-vs-FP64 distance between 1.5e-8 and 2.5e-7);
-  * the rank-one instance forced onto the bf16x3 path: there the CG runs on the
+  * the rank-one instance forced onto the bf16 Gram path: there the CG runs on the
     exact operator from the first iteration, where the FP64 path starts with
     Jacobi-CG solves accurate to 1e-6, and the two end 5e-7 apart in y
-    (measured: FP64 GPU vs reference HDSDP 3.8e-8, forced bf16x3 vs reference
+    (measured: FP64 GPU vs reference HDSDP 3.8e-8, forced bf16 vs reference
     5.1e-7)."""
 import numpy as np
 import pytest
@@ -33,7 +33,7 @@ from hdsdp_amd.solver import run_solver
 
 pytestmark = pytest.mark.gpu
 
-# name -> (instance, Gram-built Schur matrix, HDSDP_GRAM_BF16X3 override or None, y tolerance)
+# name -> (instance, Gram-built Schur matrix, HDSDP_GRAM_BF16 override or None, y tolerance)
 CASES = {
     "maxcut_n200": (lambda: I.maxcut(200, 0.05, 1, "pm1"), False, None, 1e-7),
     "theta_n50": (lambda: I.lovasz_theta(50, 0.3, 21), False, None, 1e-7),
@@ -41,7 +41,7 @@ CASES = {
     "dense_n120_m240": (lambda: I.dense_random(120, 240, 33), True, None, 1e-7),
     "rankone_n60_m500": (lambda: I.rank_one(60, 500, 5, 45), False, None, 1e-6),
     "rankone_n200_m2000": (lambda: I.rank_one(200, 2000, 20, 4), False, None, 1e-7),
-    "rankone_n200_m2000_bf16x3": (lambda: I.rank_one(200, 2000, 20, 4), True, "1", 1e-6),
+    "rankone_n200_m2000_bf16": (lambda: I.rank_one(200, 2000, 20, 4), True, "1", 1e-6),
 }
 
 
@@ -49,7 +49,7 @@ CASES = {
 def test_mixed_matches_fp64(gpu_lib, name, monkeypatch):
     make, gram, force, ytol = CASES[name]
     if force is not None:
-        monkeypatch.setenv("HDSDP_GRAM_BF16X3", force)
+        monkeypatch.setenv("HDSDP_GRAM_BF16", force)
     inst = make()
     opts = dict(relopt=1e-6, absopt=1e30)
     r64 = run_solver(gpu_lib, inst, precision=0, **opts)
