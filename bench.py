@@ -13,10 +13,14 @@ The JSON line also carries:
     HBM-bound: algorithmic bytes = 8 per Schur entry written (lower triangle
     of M over the sparse constraints) + the n x n FP64 S^-1 read once,
     divided by its average launch time from HIP events on the library's
-    stream during the timed steps;
+    stream during the timed steps; ``traffic`` is the calibrated FETCH_SIZE +
+    WRITE_SIZE of that kernel from profiles/theta400_schur_pairs_traffic.md;
   * ``phases``: device milliseconds per IPM iteration by phase;
   * ``cpu_baseline``: the reference HDSDP build (oracle/_ref, MKL) solving the
-    same instance on the host cores, when that build is present.
+    same instance on the host cores, when that build is present;
+  * ``adjoint`` (workload C5): the adjoint of the centred dual map after a
+    1e-8-gap solve (HDSDPAdjoint time) and its relative error against central
+    finite differences (h = 1e-6) along ``--adjoint-dirs`` seeded directions.
 """
 from __future__ import annotations
 
@@ -35,6 +39,7 @@ import numpy as np  # noqa: E402
 
 REF_RUNNER = os.path.join(ROOT, "oracle", "_ref", "hdsdp_ref_run")
 GOLDEN = os.path.join(ROOT, "tests", "golden")
+PAIR_TRAFFIC = os.path.join(ROOT, "profiles", "theta400_schur_pairs_traffic.md")
 
 # workload -> (description, instance constructor, reference fixture or None)
 WORKLOADS = {
@@ -63,7 +68,42 @@ def parse():
     ap.add_argument("--maxiter", type=int, default=None, help="cap IPM iterations (timing samples)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--adjoint-dirs", type=int, default=20, help="C5: finite-difference directions (0: none)")
     return ap.parse_args()
+
+
+def pair_traffic(workload):
+    """Calibrated HBM bytes per launch of the pair kernel (C2 only), from the
+    counter runs summarised in profiles/."""
+    if workload != "C2" or not os.path.exists(PAIR_TRAFFIC):
+        return None
+    with open(PAIR_TRAFFIC) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
+def adjoint_report(lib, inst, ndirs):
+    """Adjoint of the centred map after a 1e-8-gap FP64 solve, checked by central FD."""
+    from hdsdp_amd.adjoint import fd_check
+
+    t0 = time.perf_counter()
+    r = fd_check(lib, inst, ndirs, seed=5, h=1e-6, relopt=1e-8,
+                 log=lambda k, e: print(f"[bench] adjoint FD direction {k}: rel err {e:.2e}", file=sys.stderr,
+                                        flush=True))
+    errs = r["errs"]
+    return {
+        "map": "centred dual y_mu(b, C) at the final mu of a 1e-8-gap FP64 solve",
+        "loss": "||y_mu - y_t||^2, y_t ~ N(0, 1) seeded",
+        "mu": r["mu"],
+        "adjoint_ms": r["adjoint_s"] * 1e3,
+        "fd_h": 1e-6,
+        "fd_directions": ndirs,
+        "fd_max_rel_err": max(errs) if errs else None,
+        "fd_median_rel_err": float(np.median(errs)) if errs else None,
+        "check_s": time.perf_counter() - t0,
+    }
 
 
 def parity(res, fixture):
@@ -202,7 +242,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": pair_traffic(args.workload),
             "bytes_per_launch": pair_bytes,
             "avg_launch_us": pair_avg_s * 1e6,
             "launches": pair_cnt,
@@ -226,6 +266,9 @@ def main():
     }
     if not args.no_cpu_baseline and inst.factor is None:
         line["cpu_baseline"] = cpu_baseline(inst, args.cpu_threads)
+    if args.workload == "C5" and args.adjoint_dirs > 0 and prec == 0:
+        print("[bench] adjoint check", file=sys.stderr, flush=True)
+        line["adjoint"] = adjoint_report(lib, inst, args.adjoint_dirs)
     print(json.dumps(line))
 
 

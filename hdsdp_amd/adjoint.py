@@ -50,6 +50,48 @@ def centre_at(lib, inst: I.SDPInstance, mu: float, y0: np.ndarray | None = None,
     return y
 
 
+def fd_check(lib, inst: I.SDPInstance, n_dirs: int, *, seed: int = 0, h: float = 1e-6, relopt: float = 1e-8,
+             log=None, **solve_opts):
+    """Adjoint of the centred map against central finite differences.
+
+    Solves ``inst`` to ``relopt``, centres at the final mu and takes the gradient of
+    L = ||y_mu - y_t||^2 (y_t seeded normal) by HDSDPAdjoint.  Along ``n_dirs`` seeded
+    random directions (db, dC), dC symmetric, it compares <dL/db, db> + <dL/dC, dC>
+    with (L(+h) - L(-h)) / 2h, where L(+-h) centres a solve of the perturbed data at
+    the same mu.  ``log(k, err)`` is called after each direction.  Returns a dict:
+    relative errors per direction, mu, the gradients, y_mu and the seconds of the
+    adjoint call."""
+    import time
+
+    rng = np.random.default_rng(seed)
+    opts = dict(relopt=relopt, absopt=1e30, **solve_opts)
+    sv, mu, y, _ = centred_solve(lib, inst, **opts)
+    try:
+        yt = rng.standard_normal(inst.m)
+        t0 = time.perf_counter()
+        db, dC = sv.adjoint(mu, y, 2.0 * (y - yt))
+        t_adj = time.perf_counter() - t0
+    finally:
+        sv.close()
+    C0 = inst.dense_coeff(0)
+    errs = []
+    for _ in range(n_dirs):
+        dbv = rng.standard_normal(inst.m)
+        dCv = rng.standard_normal((inst.n, inst.n))
+        dCv = 0.5 * (dCv + dCv.T)
+        Lv = []
+        for sgn in (1.0, -1.0):
+            ip = I.with_data(inst, b=inst.b + sgn * h * dbv, C=C0 + sgn * h * dCv)
+            yp = centre_at(lib, ip, mu, **opts)
+            Lv.append(float(np.sum((yp - yt) ** 2)))
+        fd = (Lv[0] - Lv[1]) / (2.0 * h)
+        an = float(db @ dbv + np.sum(dC * dCv))
+        errs.append(abs(fd - an) / max(abs(fd), 1e-300))
+        if log:
+            log(len(errs), errs[-1])
+    return {"errs": errs, "mu": mu, "dLdb": db, "dLdC": dC, "y": y, "adjoint_s": t_adj}
+
+
 class SDPDual(torch.autograd.Function):
     """``y = SDPDual.apply(b, C, inst, opts)``: the centred dual solution of
     ``inst`` with its right-hand side and objective replaced by ``b`` (m) and

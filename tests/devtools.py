@@ -307,36 +307,8 @@ def parity_report(fix, res):
 
 
 def adjoint_fd_check(lib, inst, n_dirs, seed=0, h=1e-6, relopt=1e-8, **solve_opts):
-    """Adjoint of the centred map y_mu(b, C) against central finite differences.
+    """hdsdp_amd.adjoint.fd_check, returned as (relative errors, mu, dL/db, dL/dC, y_mu)."""
+    from hdsdp_amd.adjoint import fd_check
 
-    Solves ``inst`` to ``relopt``, centres at the final mu, and takes the gradient of
-    L = ||y_mu - y_t||^2 (y_t seeded normal) by HDSDPAdjoint.  Along ``n_dirs`` seeded
-    random directions (db, dC) (dC symmetric), compares <dL/db, db> + <dL/dC, dC>
-    with (L(+h) - L(-h)) / 2h, where L(+-h) centres a solve of the perturbed data at
-    the same mu.  Returns (list of relative errors, mu, dL/db, dL/dC, y_mu)."""
-    from hdsdp_amd import instances as I
-    from hdsdp_amd.adjoint import centre_at, centred_solve
-
-    rng = np.random.default_rng(seed)
-    opts = dict(relopt=relopt, absopt=1e30, **solve_opts)
-    sv, mu, y, res = centred_solve(lib, inst, **opts)
-    try:
-        yt = rng.standard_normal(inst.m)
-        db, dC = sv.adjoint(mu, y, 2.0 * (y - yt))
-    finally:
-        sv.close()
-    C0 = inst.dense_coeff(0)
-    errs = []
-    for _ in range(n_dirs):
-        dbv = rng.standard_normal(inst.m)
-        dCv = rng.standard_normal((inst.n, inst.n))
-        dCv = 0.5 * (dCv + dCv.T)
-        Lv = []
-        for sgn in (1.0, -1.0):
-            ip = I.with_data(inst, b=inst.b + sgn * h * dbv, C=C0 + sgn * h * dCv)
-            yp = centre_at(lib, ip, mu, **opts)
-            Lv.append(float(np.sum((yp - yt) ** 2)))
-        fd = (Lv[0] - Lv[1]) / (2.0 * h)
-        an = float(db @ dbv + np.sum(dC * dCv))
-        errs.append(abs(fd - an) / max(abs(fd), 1e-300))
-    return errs, mu, db, dC, y
+    r = fd_check(lib, inst, n_dirs, seed=seed, h=h, relopt=relopt, **solve_opts)
+    return r["errs"], r["mu"], r["dLdb"], r["dLdC"], r["y"]
