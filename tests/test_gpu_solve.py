@@ -19,6 +19,8 @@ import numpy as np
 import pytest
 
 import devtools as dt
+from hdsdp_amd import instances as I
+from hdsdp_amd.solver import run_solver
 
 pytestmark = pytest.mark.gpu
 
@@ -79,3 +81,18 @@ def test_solution_checks(gpu_lib):
     inst, fix, res = dt.solve_like_golden(gpu_lib, "c1_default")
     for got, ref in zip(res.dimacs, fix["dimacs"]):
         assert got <= max(10 * abs(ref), 1e-9)
+
+
+def test_setup_then_optimize_is_optimize(gpu_lib):
+    """HDSDPSetup (pre-solve and upload only) followed by HDSDPOptimize runs the same
+    solve as HDSDPOptimize alone: same iterations, bit-identical y (the kernels sum
+    in fixed orders)."""
+    from hdsdp_amd.solver import Solver
+
+    inst = I.lovasz_theta(50, 0.3, 21)
+    r0 = run_solver(gpu_lib, inst, relopt=1e-6, absopt=1e30)
+    with Solver(gpu_lib, inst, relopt=1e-6, absopt=1e30) as sv:
+        sv.setup()
+        r1 = sv.optimize()
+    assert (r1.status, r1.iters) == (r0.status, r0.iters)
+    assert np.array_equal(r1.y, r0.y)
