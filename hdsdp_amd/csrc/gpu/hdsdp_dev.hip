@@ -37,7 +37,7 @@ struct hdsdp_dev {
     int r1Gram, dnGram, nDn, r1Identity;
     double *H1;              // n x (nR1 + 1): W F
     double *Kr;              // nR1 x nR1 when the slot -> row map is not the identity
-    double *Tall;            // nDn x n^2
+    double *Tall, *Tdg;      // packed strictly-lower T_p (K1 = n (n - 1) / 2 each), diagonals (n each)
     double *Dbuf, *Ubuf;     // gramBatch x n^2
     double *Gpart;           // gramParts x nDn^2
     int gramBatch, gramParts;
@@ -45,7 +45,7 @@ struct hdsdp_dev {
     // mixed precision Gram builds (bf16 split products): FP32 copies and products
     float *LiTf;             // n x n, L^-T of the dual factor
     float *Ff, *Gf;          // n x nR1: rank-one factors and G = L^-1 F
-    float *Tf, *Dbuff, *Vf;  // dense: T_p (nDn x n^2), batch buffers
+    float *Tf, *Tdgf, *Dbuff, *Vf;  // mixed: packed T_p and diagonals, batch buffers
     // adaptive precision: the Gram blocks of M come from bf16 split products while
     // schurMixed; when the system they form cannot be factored or solved, they are
     // rebuilt in FP64 from the S^-1 and L^-1 of their build (Wm, Lim), the diagonal
@@ -496,32 +496,38 @@ int cg_solve(hdsdp_dev *d) {
 }
 
 // M(p, q) = <T_p, T_q> over the dense constraints with T_p = L^-1 A_p L^-T (so that
-// <A_p, W A_q W> = <T_p, T_q> for W = L^-T L^-1): batched GEMMs form T, a split-K
-// Gram GEMM accumulates partial blocks that are summed in a fixed order.
+// <A_p, W A_q W> = <T_p, T_q> for W = L^-T L^-1).  T_p is symmetric, so only its lower
+// triangle is formed, and <T_p, T_q> = 2 <off_p, off_q> + <dg_p, dg_q> over the packed
+// strictly-lower parts and the diagonals: a split-K Gram product over n (n - 1) / 2
+// with alpha = 2 (exact) and one over n, summed in a fixed order.
 void dense_gram(hdsdp_dev *d, const double *Li) {
     const int n = d->n, r = d->nDn, m = d->m;
-    const long nn = (long) n * n;
+    const long nn = (long) n * n, K1 = (long) n * (n - 1) / 2;
     for (int b0 = 0; b0 < r; b0 += d->gramBatch) {
         const int cnt = std::min(d->gramBatch, r - b0);
         hdk::unpack_dense(d->st, d->c, b0, cnt, d->Dbuf);
-        // U = L^-1 A_p, then T_p = U L^-T = L^-1 U^T (T_p symmetric): both products have
-        // the lower-triangular L^-1 on the left, whose zero k ranges are skipped
+        // U = L^-1 A_p, then the lower triangle of T_p = U L^-T = L^-1 U^T: both products
+        // have the lower-triangular L^-1 on the left, whose zero k ranges are skipped
         hdk::gemm<double>(d->st, false, false, false, n, n, n, 1.0, Li, n, 0, d->Dbuf, n, nn, 0.0, d->Ubuf, n, nn, cnt,
                           2);
-        hdk::gemm<double>(d->st, false, true, false, n, n, n, 1.0, Li, n, 0, d->Ubuf, n, nn, 0.0,
-                          d->Tall + (size_t) b0 * nn, n, nn, cnt, 2);
+        hdk::gemm<double>(d->st, false, true, true, n, n, n, 1.0, Li, n, 0, d->Ubuf, n, nn, 0.0, d->Dbuf, n, nn, cnt,
+                          2);
+        hdk::pack_lower<double>(d->st, n, cnt, d->Dbuf, nn, d->Tall + (size_t) b0 * K1, d->Tdg + (size_t) b0 * n);
     }
     const long kc = d->gramChunk;
-    const int full = (int) (nn / kc);
-    const long rem = nn - (long) full * kc;
+    const int full = (int) (K1 / kc);
+    const long rem = K1 - (long) full * kc;
     const long rr = (long) r * r;
+    const int nOff = full + (rem > 0 ? 1 : 0);
     if (full > 0)
-        hdk::gemm<double>(d->st, true, false, true, r, r, (int) kc, 1.0, d->Tall, (int) nn, kc, d->Tall, (int) nn, kc,
+        hdk::gemm<double>(d->st, true, false, true, r, r, (int) kc, 2.0, d->Tall, (int) K1, kc, d->Tall, (int) K1, kc,
                           0.0, d->Gpart, r, rr, full);
     if (rem > 0)
-        hdk::gemm<double>(d->st, true, false, true, r, r, (int) rem, 1.0, d->Tall + (long) full * kc, (int) nn, 0,
-                          d->Tall + (long) full * kc, (int) nn, 0, 0.0, d->Gpart + (long) full * rr, r, 0, 1);
-    hdk::gram_reduce(d->st, r, full + (rem > 0 ? 1 : 0), d->Gpart, r, rr, d->c.denseCon, d->M, m);
+        hdk::gemm<double>(d->st, true, false, true, r, r, (int) rem, 2.0, d->Tall + (long) full * kc, (int) K1, 0,
+                          d->Tall + (long) full * kc, (int) K1, 0, 0.0, d->Gpart + (long) full * rr, r, 0, 1);
+    hdk::gemm<double>(d->st, true, false, true, r, r, n, 1.0, d->Tdg, n, 0, d->Tdg, n, 0, 0.0,
+                      d->Gpart + (long) nOff * rr, r, 0, 1);
+    hdk::gram_reduce(d->st, r, nOff + 1, d->Gpart, r, rr, d->c.denseCon, d->M, m);
 }
 
 // Rebuild the Gram blocks of M in FP64 from the factor of their build, re-apply the
@@ -539,8 +545,9 @@ hdsdp_retcode rebuild_gram_fp64(hdsdp_dev *d) {
     }
     if (d->dnGram) {
         if (!d->Tall) {
-            const size_t nn = (size_t) n * n;
-            DCALL(dalloc(d, &d->Tall, nn * d->nDn));
+            const size_t nn = (size_t) n * n, K1 = (size_t) n * (n - 1) / 2;
+            DCALL(dalloc(d, &d->Tall, K1 * d->nDn));
+            DCALL(dalloc(d, &d->Tdg, (size_t) n * d->nDn));
             DCALL(dalloc(d, &d->Dbuf, nn * d->gramBatch));
             DCALL(dalloc(d, &d->Ubuf, nn * d->gramBatch));
         }
@@ -562,26 +569,32 @@ hdsdp_retcode rebuild_gram_fp64(hdsdp_dev *d) {
 //   V_p = A_p L^-T = (A_p)^T (L^-T),  T_p = L^-1 V_p = (L^-T)^T V_p,  M = T^T T.
 void dense_gram_mixed(hdsdp_dev *d) {
     const int n = d->n, r = d->nDn, m = d->m;
-    const long nn = (long) n * n;
+    const long nn = (long) n * n, K1 = (long) n * (n - 1) / 2;
     hdk::d2f(d->st, nn, d->LiT[0], d->LiTf);
     for (int b0 = 0; b0 < r; b0 += d->gramBatch) {
         const int cnt = std::min(d->gramBatch, r - b0);
         hdk::unpack_dense_f(d->st, d->c, b0, cnt, d->Dbuff);
-        hdk::gemm_bf16s_tn<float>(d->st, BF16_TERMS, false, n, n, n, 1.0, d->Dbuff, n, nn, d->LiTf, n, 0, 0.0, d->Vf, n, nn, cnt);
-        hdk::gemm_bf16s_tn<float>(d->st, BF16_TERMS, false, n, n, n, 1.0, d->LiTf, n, 0, d->Vf, n, nn, 0.0,
-                                   d->Tf + (size_t) b0 * nn, n, nn, cnt);
+        hdk::gemm_bf16s_tn<float>(d->st, BF16_TERMS, false, n, n, n, 1.0, d->Dbuff, n, nn, d->LiTf, n, 0, 0.0, d->Vf, n,
+                                  nn, cnt);
+        // lower triangle of T_p only (symmetric), into the free batch buffer, then packed
+        hdk::gemm_bf16s_tn<float>(d->st, BF16_TERMS, true, n, n, n, 1.0, d->LiTf, n, 0, d->Vf, n, nn, 0.0, d->Dbuff, n,
+                                  nn, cnt);
+        hdk::pack_lower<float>(d->st, n, cnt, d->Dbuff, nn, d->Tf + (size_t) b0 * K1, d->Tdgf + (size_t) b0 * n);
     }
     const long kc = d->gramChunk;
-    const int full = (int) (nn / kc);
-    const long rem = nn - (long) full * kc;
+    const int full = (int) (K1 / kc);
+    const long rem = K1 - (long) full * kc;
     const long rr = (long) r * r;
+    const int nOff = full + (rem > 0 ? 1 : 0);
     if (full > 0)
-        hdk::gemm_bf16s_tn<double>(d->st, BF16_TERMS, true, r, r, (int) kc, 1.0, d->Tf, (int) nn, kc, d->Tf, (int) nn, kc, 0.0,
-                                    d->Gpart, r, rr, full);
+        hdk::gemm_bf16s_tn<double>(d->st, BF16_TERMS, true, r, r, (int) kc, 2.0, d->Tf, (int) K1, kc, d->Tf, (int) K1,
+                                   kc, 0.0, d->Gpart, r, rr, full);
     if (rem > 0)
-        hdk::gemm_bf16s_tn<double>(d->st, BF16_TERMS, true, r, r, (int) rem, 1.0, d->Tf + (long) full * kc, (int) nn, 0,
-                                    d->Tf + (long) full * kc, (int) nn, 0, 0.0, d->Gpart + (long) full * rr, r, 0, 1);
-    hdk::gram_reduce(d->st, r, full + (rem > 0 ? 1 : 0), d->Gpart, r, rr, d->c.denseCon, d->M, m);
+        hdk::gemm_bf16s_tn<double>(d->st, BF16_TERMS, true, r, r, (int) rem, 2.0, d->Tf + (long) full * kc, (int) K1, 0,
+                                   d->Tf + (long) full * kc, (int) K1, 0, 0.0, d->Gpart + (long) full * rr, r, 0, 1);
+    hdk::gemm_bf16s_tn<double>(d->st, BF16_TERMS, true, r, r, n, 1.0, d->Tdgf, n, 0, d->Tdgf, n, 0, 0.0,
+                               d->Gpart + (long) nOff * rr, r, 0, 1);
+    hdk::gram_reduce(d->st, r, nOff + 1, d->Gpart, r, rr, d->c.denseCon, d->M, m);
 }
 
 // Mixed precision rank-one Gram: G = L^-1 F = (L^-T)^T F, K = G^T G, M = s s^T o K o K.
@@ -854,10 +867,10 @@ hdsdp_retcode hdsdp_dev_load(hdsdp_dev *d, const hdsdp_coeff_view *cv) {
             DCALL(dalloc(d, &d->splitW, d->splitLen));
         }
     }
-    d->H1 = d->Kr = d->Tall = d->Dbuf = d->Ubuf = d->Gpart = nullptr;
+    d->H1 = d->Kr = d->Tall = d->Tdg = d->Dbuf = d->Ubuf = d->Gpart = nullptr;
     if (c.nR1 > 0 || c.cDSR1) DCALL(dalloc(d, &d->H1, (size_t) n * (c.nR1 + 1)));
     if (d->r1Gram && !d->r1Identity) DCALL(dalloc(d, &d->Kr, (size_t) c.nR1 * c.nR1));
-    d->LiTf = d->Ff = d->Gf = d->Tf = d->Dbuff = d->Vf = nullptr;
+    d->LiTf = d->Ff = d->Gf = d->Tf = d->Tdgf = d->Dbuff = d->Vf = nullptr;
     d->Wm = d->Lim = d->mDiag0 = d->mDiagCur = d->xCoef = nullptr;
     // bf16-split Gram blocks need the matrix-free FP64 product in the refinement.  For dense
     // constraints it costs two passes over the packed coefficients and two n^3 products,
@@ -892,24 +905,28 @@ hdsdp_retcode hdsdp_dev_load(hdsdp_dev *d, const hdsdp_coeff_view *cv) {
         // goes into one batched launch (a remainder part runs alone, on too few tiles)
         long tiles = (long) ((r + 127) / 128) * ((r + 127) / 128 + 1) / 2;
         long want = std::min<long>(16, std::max<long>(1, (2048 + tiles - 1) / tiles));
-        long K = (long) nn, parts = want;
+        long K = (long) n * (n - 1) / 2, parts = want;
         for (long p = want; p <= 2 * want; ++p)
             if (K % p == 0 && K / p >= 256) {
                 parts = p;
                 break;
             }
-        d->gramChunk = (K % parts == 0) ? K / parts : std::max<long>(256, (K + parts - 1) / parts);
+        d->gramChunk = std::max<long>(1, (K % parts == 0) ? K / parts : std::max<long>(256, (K + parts - 1) / parts));
         d->gramParts = (int) ((K + d->gramChunk - 1) / d->gramChunk);
+        const size_t K1 = std::max<size_t>(1, (size_t) n * (n - 1) / 2);
         if (mixedGram) {
-            DCALL(dalloc(d, &d->Tf, nn * r));
+            DCALL(dalloc(d, &d->Tf, K1 * r));
+            DCALL(dalloc(d, &d->Tdgf, (size_t) n * r));
             DCALL(dalloc(d, &d->Dbuff, nn * d->gramBatch));
             DCALL(dalloc(d, &d->Vf, nn * d->gramBatch));
         } else {
-            DCALL(dalloc(d, &d->Tall, nn * r));
+            DCALL(dalloc(d, &d->Tall, K1 * r));
+            DCALL(dalloc(d, &d->Tdg, (size_t) n * r));
             DCALL(dalloc(d, &d->Dbuf, nn * d->gramBatch));
             DCALL(dalloc(d, &d->Ubuf, nn * d->gramBatch));
         }
-        DCALL(dalloc(d, &d->Gpart, (size_t) r * r * d->gramParts));
+        // split-K parts of the off-diagonal Gram and one for the diagonal Gram
+        DCALL(dalloc(d, &d->Gpart, (size_t) r * r * (d->gramParts + 1)));
     }
     return check_launch();
 }

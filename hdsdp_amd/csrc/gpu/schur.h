@@ -62,6 +62,10 @@ void r1_square(hipStream_t s, const DevCoeffs &c, const double *K, long ldk, dou
 // full n x n copies of dense slots slot0 .. slot0 + cnt - 1
 void unpack_dense(hipStream_t s, const DevCoeffs &c, int slot0, int cnt, double *out);
 void unpack_dense_f(hipStream_t s, const DevCoeffs &c, int slot0, int cnt, float *out);
+// T_p (cnt matrices n x n, lower valid, stride strideIn) -> strictly lower part packed by
+// columns (off + p K1, K1 = n (n - 1) / 2) and diagonal (dg + p n)
+template <typename T>
+void pack_lower(hipStream_t s, int n, int cnt, const T *Tin, long strideIn, T *off, T *dg);
 // M(denseCon[p] - 1, denseCon[q] - 1) = sum_b P_b(p, q), p >= q
 void gram_reduce(hipStream_t s, int r, int nPart, const double *P, long ldp, long strideP, const int *denseCon,
                  double *M, long ldm);

@@ -320,6 +320,31 @@ __global__ void gram_reduce_kernel(int r, int nPart, const double *P, long ldp, 
     M[i + j * ldm] = v;
 }
 
+// Strictly lower part of each symmetric T_p (n x n, lower triangle valid) packed column
+// by column into off[p * K1 + j n - j (j + 1) / 2 + (i - j - 1)], i > j, and its diagonal
+// into dg[p * n + i].
+template <typename T>
+__global__ void pack_lower_kernel(int n, const T *Tin, long strideIn, T *off, long K1, T *dg) {
+    const int p = blockIdx.y;
+    const long nn = (long) n * n;
+    const T *src = Tin + p * strideIn;
+    for (long e = blockIdx.x * (long) blockDim.x + threadIdx.x; e < nn; e += (long) gridDim.x * blockDim.x) {
+        const long i = e % n, j = e / n;
+        if (i > j) off[p * K1 + j * n - j * (j + 1) / 2 + (i - j - 1)] = src[e];
+        else if (i == j) dg[(long) p * n + i] = src[e];
+    }
+}
+
+template <typename T>
+void pack_lower(hipStream_t s, int n, int cnt, const T *Tin, long strideIn, T *off, T *dg) {
+    if (cnt <= 0) return;
+    const long K1 = (long) n * (n - 1) / 2;
+    pack_lower_kernel<T><<<dim3(1024, cnt), 256, 0, s>>>(n, Tin, strideIn, off, K1, dg);
+}
+
+template void pack_lower<double>(hipStream_t, int, int, const double *, long, double *, double *);
+template void pack_lower<float>(hipStream_t, int, int, const float *, long, float *, float *);
+
 __global__ void add_diag_delta_kernel(int r, const int *con, const double *cur, const double *ref, double *M,
                                       long ldm) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
