@@ -36,6 +36,8 @@ CASES = {
     "c5_gap1e-6": (1e-7, 1e-5, False),
     "dense_n120_m240": (1e-7, 1e-5, False),
     "rankone_n60_m500": (1e-7, 1e-5, False),
+    "dense_n200_m400": (1e-7, 1e-5, False),
+    "rankone_n200_m2000": (1e-7, 1e-5, False),
 }
 
 
