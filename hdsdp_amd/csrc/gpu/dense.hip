@@ -51,10 +51,16 @@ __device__ inline void tri_krange(int tri, int i0, int j0, int tile, int K, int 
 // y <= x: consecutive workgroups go to consecutive XCDs, so every XCD gets the same
 // share of the triangle (a 2-D grid gives tile column x to XCD x mod 8, and the
 // columns near the diagonal end hold unequal work).
-__device__ inline void tile_xy(int lin, int &bx, int &by) {
+//
+// With tri == 2 (op(A) lower: the work of a tile grows with its row) the tile row of
+// a 2-D grid is rotated by the other grid coordinates: workgroup x + gridDim.x (y + ...)
+// goes to XCD (x + ...) mod 8, so with 8k tile rows XCD x would receive every tile of
+// row x (the heaviest rows all on one XCD); the rotation gives each XCD every row.
+__device__ inline void tile_xy(int lin, int &bx, int &by, int tri = 0) {
     if (!lin) {
         bx = blockIdx.x;
         by = blockIdx.y;
+        if (tri == 2) bx = (int) ((blockIdx.x + blockIdx.y + blockIdx.z) % gridDim.x);
         return;
     }
     const int t = blockIdx.x;
@@ -71,7 +77,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha,
                                                    T *C, int ldc, long sC, int tri, int lin) {
     typedef typename Mfma<T>::acc_t acc_t;
     int bx, by;
-    tile_xy(lin, bx, by);
+    tile_xy(lin, bx, by, tri);
     const int i0 = bx * TILE, j0 = by * TILE;
     if (LOWER && i0 + TILE - 1 < j0) return;
     int kLo, kHi;
@@ -263,7 +269,7 @@ __global__ __launch_bounds__(256) void gemm_big_kernel(int M, int N, int K, T al
                                                        long sC, int tri, int lin) {
     typedef typename Mfma<T>::acc_t acc_t;
     int bx, by;
-    tile_xy(lin, bx, by);
+    tile_xy(lin, bx, by, tri);
     const int i0 = bx * BT, j0 = by * BT;
     if (LOWER && i0 + BT - 1 < j0) return;
     int kLo, kHi;
