@@ -73,13 +73,19 @@ __device__ inline void store16(__bf16 *p0, __bf16 *p1, __bf16 *p2, const float (
     }
 }
 
+// tri: 1 when column j of Y is zero below row j (Y(k, j) = 0 for k > j), 2 when column i of
+// X is (X(k, i) = 0 for k > i): the k range of a tile ends at its last column (row); only
+// exact zeros are skipped.  With tri == 2 the tile row of a 2-D grid is rotated by the
+// other grid coordinates, so that every XCD gets every row (workgroup x + gridDim.x
+// (y + ...) goes to XCD (x + ...) mod 8, and the work of a tile grows with its row).
 template <typename Tout, bool LOWER, int NT>
 __global__ __launch_bounds__(256) void gemm_bf16s_tn_kernel(int M, int N, int K, double alpha, const float *X, int ldx,
                                                             long sX, const float *Y, int ldy, long sY, double beta,
-                                                            Tout *C, int ldc, long sC, int lin) {
+                                                            Tout *C, int ldc, long sC, int lin, int tri) {
     // lin: blockIdx.x runs over the lower tiles only (t = x (x + 1) / 2 + y, y <= x), so
     // that consecutive workgroups, dealt to consecutive XCDs, share the triangle evenly
     int bx = blockIdx.x, by = blockIdx.y;
+    if (!lin && tri == 2) bx = (int) ((blockIdx.x + blockIdx.y + blockIdx.z) % gridDim.x);
     if (lin) {
         const int t = blockIdx.x;
         int x = (int) ((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
@@ -90,6 +96,7 @@ __global__ __launch_bounds__(256) void gemm_bf16s_tn_kernel(int M, int N, int K,
     }
     const int i0 = bx * XT, j0 = by * XT;
     if (LOWER && i0 + XT - 1 < j0) return;
+    const int Ke = (tri == 1) ? min(K, j0 + XT) : (tri == 2) ? min(K, i0 + XT) : K;
     X += (long) blockIdx.z * sX;
     Y += (long) blockIdx.z * sY;
     C += (long) blockIdx.z * sC;
@@ -116,13 +123,13 @@ __global__ __launch_bounds__(256) void gemm_bf16s_tn_kernel(int M, int N, int K,
 
     float vx[16], vy[16];
     auto fetch = [&](int k0) {
-        const bool interior = (i0 + XT <= M) && (j0 + XT <= N) && (k0 + XK <= K);
+        const bool interior = (i0 + XT <= M) && (j0 + XT <= N) && (k0 + XK <= Ke);
         if (interior) {
-            load16<false>(X, ldx, M, K, i0 + lrow, k0 + lkh, vx);
-            load16<false>(Y, ldy, N, K, j0 + lrow, k0 + lkh, vy);
+            load16<false>(X, ldx, M, Ke, i0 + lrow, k0 + lkh, vx);
+            load16<false>(Y, ldy, N, Ke, j0 + lrow, k0 + lkh, vy);
         } else {
-            load16<true>(X, ldx, M, K, i0 + lrow, k0 + lkh, vx);
-            load16<true>(Y, ldy, N, K, j0 + lrow, k0 + lkh, vy);
+            load16<true>(X, ldx, M, Ke, i0 + lrow, k0 + lkh, vx);
+            load16<true>(Y, ldy, N, Ke, j0 + lrow, k0 + lkh, vy);
         }
     };
     auto stash = [&](int buf) {
@@ -133,7 +140,7 @@ __global__ __launch_bounds__(256) void gemm_bf16s_tn_kernel(int M, int N, int K,
     fetch(0);
     stash(0);
     __syncthreads();
-    const int nslab = (K + XK - 1) / XK;
+    const int nslab = (Ke + XK - 1) / XK;
     for (int sl = 0; sl < nslab; ++sl) {
         const int cur = sl & 1;
         const bool more = sl + 1 < nslab;
@@ -200,33 +207,34 @@ __global__ __launch_bounds__(256) void gemm_bf16s_tn_kernel(int M, int N, int K,
 
 template <typename Tout, int NT>
 void launch(hipStream_t s, bool lower, int M, int N, int K, double alpha, const float *X, int ldx, long sX,
-            const float *Y, int ldy, long sY, double beta, Tout *C, int ldc, long sC, int batch) {
+            const float *Y, int ldy, long sY, double beta, Tout *C, int ldc, long sC, int batch, int tri) {
     const int mt = (M + XT - 1) / XT, nt = (N + XT - 1) / XT;
     const int lin = (lower && mt == nt) ? 1 : 0;
     dim3 grid(lin ? mt * (mt + 1) / 2 : mt, lin ? 1 : nt, batch);
     if (lower)
         gemm_bf16s_tn_kernel<Tout, true, NT><<<grid, 256, 0, s>>>(M, N, K, alpha, X, ldx, sX, Y, ldy, sY, beta, C, ldc,
-                                                                  sC, lin);
+                                                                  sC, lin, tri);
     else
         gemm_bf16s_tn_kernel<Tout, false, NT><<<grid, 256, 0, s>>>(M, N, K, alpha, X, ldx, sX, Y, ldy, sY, beta, C,
-                                                                   ldc, sC, 0);
+                                                                   ldc, sC, 0, tri);
 }
 
 }  // namespace
 
 template <typename Tout>
 void gemm_bf16s_tn(hipStream_t s, int terms, bool lower, int M, int N, int K, double alpha, const float *X, int ldx,
-                   long sX, const float *Y, int ldy, long sY, double beta, Tout *C, int ldc, long sC, int batch) {
+                   long sX, const float *Y, int ldy, long sY, double beta, Tout *C, int ldc, long sC, int batch,
+                   int tri) {
     if (M <= 0 || N <= 0 || batch <= 0) return;
     if (terms == 3)
-        launch<Tout, 3>(s, lower, M, N, K, alpha, X, ldx, sX, Y, ldy, sY, beta, C, ldc, sC, batch);
+        launch<Tout, 3>(s, lower, M, N, K, alpha, X, ldx, sX, Y, ldy, sY, beta, C, ldc, sC, batch, tri);
     else
-        launch<Tout, 2>(s, lower, M, N, K, alpha, X, ldx, sX, Y, ldy, sY, beta, C, ldc, sC, batch);
+        launch<Tout, 2>(s, lower, M, N, K, alpha, X, ldx, sX, Y, ldy, sY, beta, C, ldc, sC, batch, tri);
 }
 
 template void gemm_bf16s_tn<float>(hipStream_t, int, bool, int, int, int, double, const float *, int, long,
-                                   const float *, int, long, double, float *, int, long, int);
+                                   const float *, int, long, double, float *, int, long, int, int);
 template void gemm_bf16s_tn<double>(hipStream_t, int, bool, int, int, int, double, const float *, int, long,
-                                    const float *, int, long, double, double *, int, long, int);
+                                    const float *, int, long, double, double *, int, long, int, int);
 
 }  // namespace hdk

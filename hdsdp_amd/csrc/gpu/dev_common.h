@@ -48,9 +48,11 @@ void gemm_splitk(hipStream_t s, bool transA, bool transB, bool lower, int M, int
 // operand split into `terms` bf16 terms (2: three products, ~16 bits; 3: six products,
 // ~24 bits, FP32-equivalent), FP32 accumulation flushed into FP64 every 256 k.
 // X: K x M (ldx), Y: K x N (ldy), column-major FP32; lower restricts C to i >= j.
+// tri = 1 when Y(k, j) = 0 for k > j, 2 when X(k, i) = 0 for k > i (those k are skipped).
 template <typename Tout>
 void gemm_bf16s_tn(hipStream_t s, int terms, bool lower, int M, int N, int K, double alpha, const float *X, int ldx,
-                   long sX, const float *Y, int ldy, long sY, double beta, Tout *C, int ldc, long sC, int batch = 1);
+                   long sX, const float *Y, int ldy, long sY, double beta, Tout *C, int ldc, long sC, int batch = 1,
+                   int tri = 0);
 
 // A second stream and a ring of events for look-ahead factorisations.
 constexpr int LA_EVENTS = 16;

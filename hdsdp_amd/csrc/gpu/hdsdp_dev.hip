@@ -574,11 +574,13 @@ void dense_gram_mixed(hdsdp_dev *d) {
     for (int b0 = 0; b0 < r; b0 += d->gramBatch) {
         const int cnt = std::min(d->gramBatch, r - b0);
         hdk::unpack_dense_f(d->st, d->c, b0, cnt, d->Dbuff);
+        // V = A_p L^-T: column j of L^-T is zero below row j (tri 1)
         hdk::gemm_bf16s_tn<float>(d->st, BF16_TERMS, false, n, n, n, 1.0, d->Dbuff, n, nn, d->LiTf, n, 0, 0.0, d->Vf, n,
-                                  nn, cnt);
-        // lower triangle of T_p only (symmetric), into the free batch buffer, then packed
+                                  nn, cnt, 1);
+        // lower triangle of T_p = (L^-T)^T V only (symmetric), into the free batch buffer,
+        // then packed; column i of the left operand L^-T is zero below row i (tri 2)
         hdk::gemm_bf16s_tn<float>(d->st, BF16_TERMS, true, n, n, n, 1.0, d->LiTf, n, 0, d->Vf, n, nn, 0.0, d->Dbuff, n,
-                                  nn, cnt);
+                                  nn, cnt, 2);
         hdk::pack_lower<float>(d->st, n, cnt, d->Dbuff, nn, d->Tf + (size_t) b0 * K1, d->Tdgf + (size_t) b0 * n);
     }
     const long kc = d->gramChunk;
