@@ -665,26 +665,79 @@ template void trtri<float>(hipStream_t, int, const float *, int, float *, int, f
 
 // ---- matrix-vector ------------------------------------------------------------
 
-template <typename T>
-__global__ __launch_bounds__(256) void coldot_kernel(int m, int ncols, int tri, T alpha, const T *__restrict__ A,
-                                                     int lda, const T *__restrict__ x, T beta, T *y) {
-    const int j = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int lane = threadIdx.x & 63;
-    if (j >= ncols) return;
-    const int lo = (tri == 1) ? j : 0;
-    const int hi = (tri == 2) ? j + 1 : m;
-    const T *col = A + (long) j * lda;
-    T sum = 0;
-    for (int i = lo + lane; i < hi; i += 64) sum += col[i] * x[i];
-    for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off);
-    if (lane == 0) y[j] = alpha * sum + ((beta != (T) 0) ? beta * y[j] : (T) 0);
+// Sums over one U-row group for NC columns: lane rows i, i + 64, ..., i + (U-1) 64,
+// all 2U..(NC+1)U loads issued before the FMAs, each column's FMAs in row order.
+template <typename T, int U, int NC>
+__device__ inline void coldot_group(const T *const (&col)[NC], const T *__restrict__ x, int i, T (&sum)[NC]) {
+    T a[NC][U], b[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        b[u] = x[i + u * 64];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) a[c][u] = col[c][i + u * 64];
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int u = 0; u < U; ++u) sum[c] += a[c][u] * b[u];
 }
 
+// y_j = alpha <A(lo:hi, j), x(lo:hi)> + beta y_j with (lo, hi) = (0, m), (j, m) for tri 1
+// and (0, j + 1) for tri 2.  One wave per NC adjacent columns (NC > 1 only for tri 0 and
+// 2, where the columns share the x rows and one x load serves all of them).  Each lane
+// accumulates rows lo + lane, lo + lane + 64, ... in order and the wave adds its lanes by a
+// fixed shuffle tree, so the result does not depend on NC or on the unrolling.
+template <typename T, int NC>
+__global__ __launch_bounds__(256) void coldot_kernel(int m, int ncols, int tri, T alpha, const T *__restrict__ A,
+                                                     int lda, const T *__restrict__ x, T beta, T *y) {
+    const int j0 = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * NC;
+    const int lane = threadIdx.x & 63;
+    if (j0 >= ncols) return;
+    const T *col[NC];
+    int hic[NC];
+    T sum[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const int j = min(j0 + c, ncols - 1);  // a ragged last wave repeats the last column
+        col[c] = A + (long) j * lda;
+        hic[c] = (tri == 2) ? j + 1 : m;
+        sum[c] = 0;
+    }
+    const int hmin = hic[0];  // hic is nondecreasing in c
+    int i = ((NC == 1 && tri == 1) ? j0 : 0) + lane;
+    for (; i + 7 * 64 < hmin; i += 8 * 64) coldot_group<T, 8, NC>(col, x, i, sum);
+    if (i + 3 * 64 < hmin) {
+        coldot_group<T, 4, NC>(col, x, i, sum);
+        i += 4 * 64;
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+        for (int k = i; k < hic[c]; k += 64) sum[c] += col[c][k] * x[k];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        for (int off = 32; off > 0; off >>= 1) sum[c] += __shfl_down(sum[c], off);
+        if (lane == 0 && j0 + c < ncols) y[j0 + c] = alpha * sum[c] + ((beta != (T) 0) ? beta * y[j0 + c] : (T) 0);
+    }
+}
+
+template <typename T, int NC>
+static void coldot_launch(hipStream_t s, int m, int ncols, int tri, T alpha, const T *A, int lda, const T *x, T beta,
+                          T *y) {
+    const int waves = (ncols + NC - 1) / NC;
+    coldot_kernel<T, NC><<<(waves + 3) / 4, 256, 0, s>>>(m, ncols, tri, alpha, A, lda, x, beta, y);
+}
+
+// Columns per wave: shared x loads pay once there are enough waves to fill the chip
+// (measured on 20000^2 FP64: 4.4 TB/s at one column per wave, 5.8 TB/s at four).
 template <typename T>
 void coldot(hipStream_t s, int m, int ncols, int tri, T alpha, const T *A, int lda, const T *x, T beta, T *y) {
     if (ncols <= 0) return;
-    int blocks = (ncols + 3) / 4;
-    coldot_kernel<T><<<blocks, 256, 0, s>>>(m, ncols, tri, alpha, A, lda, x, beta, y);
+    if (tri != 1 && ncols >= 8192)
+        coldot_launch<T, 4>(s, m, ncols, tri, alpha, A, lda, x, beta, y);
+    else if (tri != 1 && ncols >= 1024)
+        coldot_launch<T, 2>(s, m, ncols, tri, alpha, A, lda, x, beta, y);
+    else
+        coldot_launch<T, 1>(s, m, ncols, tri, alpha, A, lda, x, beta, y);
 }
 
 template void coldot<double>(hipStream_t, int, int, int, double, const double *, int, const double *, double,
