@@ -25,9 +25,14 @@ INC = os.path.join(ROOT, "include")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-# Packed FP32 arithmetic is disabled for every HIP translation unit.
+# Packed FP32 arithmetic is disabled for every HIP translation unit.  MFMA takes its
+# accumulators in ordinary VGPRs: with the compiler's default AGPR form the GEMM loops
+# copied every accumulator AGPR -> VGPR -> AGPR once per k slab (a drain of the matrix
+# pipeline per slab) and needed 318 registers per lane, one wave per SIMD; the VGPR form
+# runs the same MFMA instructions in 202 registers, two waves per SIMD.
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-I" + INC, "-I" + HOST, "-I" + GPU,
-             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result",
+             "-mllvm", "-amdgpu-mfma-vgpr-form"]
 C_FLAGS = ["-std=gnu11", "-O2", "-fPIC", "-Wall", "-Wno-unused-function", "-I" + INC, "-I" + HOST]
 
 
