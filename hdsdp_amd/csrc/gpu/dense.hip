@@ -266,8 +266,9 @@ __device__ inline void big_store(T (*As)[BT + BPAD], T (*Bs)[BT + BPAD], int tid
 template <typename T, bool TA, bool TB, bool LOWER, bool VEC>
 __global__ __launch_bounds__(256) void gemm_big_kernel(int M, int N, int K, T alpha, const T *A, int lda, long sA,
                                                        const T *B, int ldb, long sB, T beta, T *C, int ldc,
-                                                       long sC, int tri, int lin) {
+                                                       long sC, int tri, int lin, int kLast) {
     typedef typename Mfma<T>::acc_t acc_t;
+    if (kLast > 0 && blockIdx.z == gridDim.z - 1) K = kLast;  // ragged last k part of a split product
     int bx, by;
     tile_xy(lin, bx, by, tri);
     const int i0 = bx * BT, j0 = by * BT;
@@ -349,6 +350,40 @@ __global__ __launch_bounds__(256) void gemm_big_kernel(int M, int N, int K, T al
     }
 }
 
+// 128-wide tile launch; batch z multiplies K-wide slices at strides sA, sB (the last
+// one kLast wide when kLast > 0)
+template <typename T>
+static void gemm_big(hipStream_t s, bool transA, bool transB, bool lower, int M, int N, int K, T alpha, const T *A,
+                     int lda, long strideA, const T *B, int ldb, long strideB, T beta, T *C, int ldc, long strideC,
+                     int batch, int tri, int kLast) {
+    const int mt = (M + BT - 1) / BT, nt = (N + BT - 1) / BT;
+    const int lin = (lower && mt == nt) ? 1 : 0;
+    dim3 bgrid(lin ? mt * (mt + 1) / 2 : mt, lin ? 1 : nt, batch);
+    // 16-byte loads need even leading dimensions, strides and addresses
+    const bool vec = sizeof(T) == 8 && lda % 2 == 0 && ldb % 2 == 0 && strideA % 2 == 0 && strideB % 2 == 0 &&
+                     ((uintptr_t) A % 16) == 0 && ((uintptr_t) B % 16) == 0;
+#define HDK_GEMM_BIG(ta, tb, lo)                                                                                 \
+    if (transA == ta && transB == tb && lower == lo) {                                                           \
+        if (vec)                                                                                                 \
+            gemm_big_kernel<T, ta, tb, lo, sizeof(T) == 8><<<bgrid, 256, 0, s>>>(                                \
+                M, N, K, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc, strideC, tri, lin, kLast);       \
+        else                                                                                                     \
+            gemm_big_kernel<T, ta, tb, lo, false><<<bgrid, 256, 0, s>>>(M, N, K, alpha, A, lda, strideA, B, ldb, \
+                                                                        strideB, beta, C, ldc, strideC, tri,     \
+                                                                        lin, kLast);                             \
+        return;                                                                                                  \
+    }
+    HDK_GEMM_BIG(false, false, false)
+    HDK_GEMM_BIG(false, true, false)
+    HDK_GEMM_BIG(true, false, false)
+    HDK_GEMM_BIG(true, true, false)
+    HDK_GEMM_BIG(false, false, true)
+    HDK_GEMM_BIG(false, true, true)
+    HDK_GEMM_BIG(true, false, true)
+    HDK_GEMM_BIG(true, true, true)
+#undef HDK_GEMM_BIG
+}
+
 template <typename T>
 void gemm(hipStream_t s, bool transA, bool transB, bool lower, int M, int N, int K, T alpha, const T *A,
           int lda, long strideA, const T *B, int ldb, long strideB, T beta, T *C, int ldc, long strideC,
@@ -358,32 +393,9 @@ void gemm(hipStream_t s, bool transA, bool transB, bool lower, int M, int N, int
     // the 128-wide tiles pay off only when the k loop is long enough to hide their
     // prologue and epilogue (measured: rank-64 updates run faster on 64-wide tiles)
     if (bigTiles >= 192 && K >= 256) {
-        const int mt = (M + BT - 1) / BT, nt = (N + BT - 1) / BT;
-        const int lin = (lower && mt == nt) ? 1 : 0;
-        dim3 bgrid(lin ? mt * (mt + 1) / 2 : mt, lin ? 1 : nt, batch);
-        // 16-byte loads need even leading dimensions, strides and addresses
-        const bool vec = sizeof(T) == 8 && lda % 2 == 0 && ldb % 2 == 0 && strideA % 2 == 0 && strideB % 2 == 0 &&
-                         ((uintptr_t) A % 16) == 0 && ((uintptr_t) B % 16) == 0;
-#define HDK_GEMM_BIG(ta, tb, lo)                                                                                \
-        if (transA == ta && transB == tb && lower == lo) {                                                      \
-            if (vec)                                                                                            \
-                gemm_big_kernel<T, ta, tb, lo, sizeof(T) == 8><<<bgrid, 256, 0, s>>>(                           \
-                    M, N, K, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc, strideC, tri, lin);         \
-            else                                                                                                \
-                gemm_big_kernel<T, ta, tb, lo, false><<<bgrid, 256, 0, s>>>(M, N, K, alpha, A, lda, strideA, B, \
-                                                                            ldb, strideB, beta, C, ldc,         \
-                                                                            strideC, tri, lin);                 \
-            return;                                                                                             \
-        }
-        HDK_GEMM_BIG(false, false, false)
-        HDK_GEMM_BIG(false, true, false)
-        HDK_GEMM_BIG(true, false, false)
-        HDK_GEMM_BIG(true, true, false)
-        HDK_GEMM_BIG(false, false, true)
-        HDK_GEMM_BIG(false, true, true)
-        HDK_GEMM_BIG(true, false, true)
-        HDK_GEMM_BIG(true, true, true)
-#undef HDK_GEMM_BIG
+        gemm_big<T>(s, transA, transB, lower, M, N, K, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc,
+                    strideC, batch, tri, 0);
+        return;
     }
     const int mt = (M + TILE - 1) / TILE, nt = (N + TILE - 1) / TILE;
     const int lin = (lower && mt == nt) ? 1 : 0;
@@ -427,13 +439,41 @@ __global__ void splitk_reduce_kernel(int M, int N, int P, T alpha, const T *W, T
     }
 }
 
+static int cu_count() {
+    static int cache[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cache[dev] <= 0) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+        cache[dev] = v;
+    }
+    return cache[dev];
+}
+
+// Parts P (k ranges of at least 1024) minimising the modelled time: the 128-wide FP64
+// kernel holds two workgroups per CU, so P parts of T output tiles take
+// ceil(T P / (2 CUs)) rounds of a k range K / P each, plus the partial-sum traffic
+// (written by the product, read by the reduction) per part.  Ties go to fewer parts.
 int splitk_parts(int M, int N, int K, bool lower) {
-    const long tiles = std::max<long>(1, (long) ((M + BT - 1) / BT) * ((N + BT - 1) / BT) / (lower ? 2 : 1));
-    if (tiles >= 192) return 1;
-    // enough 128-wide tiles for two rounds over the CUs, at least 1024 of k per part
-    long P = std::min<long>(16, (512 + tiles - 1) / tiles);
-    P = std::min<long>(P, K / 1024);
-    return (int) std::max<long>(P, 1);
+    const long mt = (M + BT - 1) / BT, nt = (N + BT - 1) / BT;
+    const long tiles = std::max<long>(1, lower ? (mt == nt ? mt * (mt + 1) / 2 : mt * nt / 2) : mt * nt);
+    const long slots = 2L * cu_count();
+    const double secPerK = 2.0 * BT * BT * (double) slots / 68e12;           // one round, one k
+    const double secPerPart = (double) M * N * 8.0 * (lower ? 1.0 : 2.0) / 5e12;  // partials out and in
+    int best = 1;
+    double tBest = 1e300;
+    for (int P = 1; P <= 32 && P <= K / 1024; ++P) {
+        const long kc = ((K + P - 1) / P + KSTEP - 1) / KSTEP * KSTEP;
+        const int parts = (int) ((K + kc - 1) / kc);
+        const long rounds = (tiles * parts + slots - 1) / slots;
+        const double t = rounds * (double) kc * secPerK + (parts > 1 ? parts * secPerPart : 0.0);
+        if (t < 0.97 * tBest) {
+            tBest = t;
+            best = parts;
+        }
+    }
+    return best;
 }
 
 template <typename T>
@@ -447,15 +487,14 @@ void gemm_splitk(hipStream_t s, bool transA, bool transB, bool lower, int M, int
         gemm<T>(s, transA, transB, lower, M, N, K, alpha, A, lda, 0, B, ldb, 0, beta, C, ldc, 0, 1, 0);
         return;
     }
-    const int kc = (int) (((K + P - 1) / P + 15) / 16 * 16);
-    const int full = K / kc, rem = K - full * kc;
+    // parts of kc (a multiple of the k slab), the last one K - (parts - 1) kc, in one launch
+    const int kc = (int) (((K + P - 1) / P + KSTEP - 1) / KSTEP * KSTEP);
+    const int parts = (K + kc - 1) / kc;
+    const int kLast = K - (parts - 1) * kc;
     const long sA = transA ? kc : (long) kc * lda, sB = transB ? (long) kc * ldb : kc;
-    gemm<T>(s, transA, transB, lower, M, N, kc, (T) 1, A, lda, sA, B, ldb, sB, (T) 0, work, M, (long) mn, full, 0);
-    if (rem > 0)
-        gemm<T>(s, transA, transB, lower, M, N, rem, (T) 1, A + full * sA, lda, 0, B + full * sB, ldb, 0, (T) 0,
-                work + full * mn, M, 0, 1, 0);
-    splitk_reduce_kernel<T><<<4096, 256, 0, s>>>(M, N, full + (rem > 0 ? 1 : 0), alpha, work, beta, C, ldc,
-                                                 lower ? 1 : 0);
+    gemm_big<T>(s, transA, transB, lower, M, N, kc, (T) 1, A, lda, sA, B, ldb, sB, (T) 0, work, M, (long) mn, parts,
+                0, kLast);
+    splitk_reduce_kernel<T><<<4096, 256, 0, s>>>(M, N, parts, alpha, work, beta, C, ldc, lower ? 1 : 0);
 }
 
 template void gemm_splitk<double>(hipStream_t, bool, bool, bool, int, int, int, double, const double *, int,

@@ -200,3 +200,21 @@ def test_dense_schur_large_tiles(gpu_lib, n, m):
     Mref = T @ T.T
     assert rel(M, Mref) < 1e-9
     assert rel(a, [np.sum(c.dense() * W) for c in coeffs[1:]]) < 1e-9
+
+
+def test_rank_one_buffer_split_k(gpu_lib):
+    """The rank-m buffer update F diag(w) F^T of m = 4000 dense rank-one constraints at
+    n = 300 runs as a k-split product (three k parts of 1344, 1344 and 1312 in one
+    launch) and an in-order reduction of the partials: S = C - A^T y + eye I against
+    the dense sum within 1e-11 relative."""
+    rng = np.random.default_rng(12)
+    coeffs = dt.rand_coeffs(rng, 300, 4000, [dt.DSR1])
+    y = 0.1 * rng.standard_normal(4000)
+    g = dt.Dev(gpu_lib, coeffs)
+    try:
+        g.update(1.0, -1.0, y, 0.7, 0)
+        S = g.get_dual()
+    finally:
+        g.close()
+    Sref = dt.dense_buffer(coeffs, 1.0, -1.0, y, 0.7)
+    assert rel(S, Sref) < 1e-11
