@@ -502,43 +502,42 @@ template void gemm_splitk<double>(hipStream_t, bool, bool, bool, int, int, int, 
 
 // ---- Cholesky ----------------------------------------------------------------
 
-// Broadcast the value that lane `src` holds (src uniform) to the whole wave.
-__device__ inline double lane_bcast(double v, int src) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int) (b & 0xffffffffLL), src);
-    const int hi = __builtin_amdgcn_readlane((int) (b >> 32), src);
-    return __longlong_as_double(((long long) (unsigned) lo) | ((long long) hi << 32));
-}
-__device__ inline float lane_bcast(float v, int src) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
-}
-
-// Column r of L^-1 for the 64 x 64 lower factor whose row q is held by lane q in
-// a[0..q] (forward substitution L x = e_r, rows of L broadcast by readlane).
+// Column r of L^-1 for the 64 x 64 lower factor held column-major in LDS,
+// Ls[c][q] = L(q, c): forward substitution L x = e_r, every lane reading the same
+// entry of column j (an LDS broadcast).
 template <typename T>
-__device__ inline void leaf_inverse(const T (&a)[NB], int r, T (&x)[NB]) {
+__device__ inline void leaf_inverse(const T (&Ls)[NB][NB], int r, T (&x)[NB]) {
+    // an opaque copy of r: the identity column is formed here rather than merged with
+    // the callers' identical identity padding (which kept 64 values live and spilled)
+    asm volatile("" : "+v"(r));
 #pragma unroll
     for (int j = 0; j < NB; ++j) x[j] = (j == r) ? (T) 1 : (T) 0;
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
-        x[j] = x[j] / lane_bcast(a[j], j);
+        x[j] = x[j] / Ls[j][j];
 #pragma unroll
-        for (int k = j + 1; k < NB; ++k) x[k] -= lane_bcast(a[j], k) * x[j];
+        for (int k = j + 1; k < NB; ++k) x[k] -= Ls[j][k] * x[j];
+        // keeps the (read-only) LDS loads of later steps from being hoisted here,
+        // which would hold thousands of values in registers and spill
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
 // One wave factors a kb x kb diagonal block (kb <= 64) and inverts its factor.
-// Lane r holds row r in registers; the column needed at step j is broadcast lane by
-// lane (readlane), so the whole factorisation runs without LDS or barriers:
+// Lane r holds row r in registers; at step j every lane stores its entry of column j
+// in LDS and reads the column back as broadcasts:
 //   step j: p = a(j, j); row r: t = a(r, j) / p; a(r, c) -= t a(c, j) for c > j;
 //           then column j is scaled by 1 / sqrt(p) and a(j, j) = sqrt(p).
 // Entries above the diagonal are scratch.  The inverse X = L^-1 follows by forward
-// substitution, lane c computing column c.  Rows and columns beyond kb are the
-// identity.  Writes L (lower) in place and X into Dinv (64 x 64, zeros above);
-// sets *info = k0 + j + 1 at the first non-positive pivot.
+// substitution from L in LDS, lane c computing column c.  Rows and columns beyond kb
+// are the identity.  Writes L (lower) in place and X into Dinv (64 x 64, zeros
+// above); sets *info = k0 + j + 1 at the first non-positive pivot.  (The LDS
+// broadcast replaced lane-to-lane reads: one v_readlane pair per operand plus hazard
+// waits, three times the instructions; the arithmetic is the same.)
 template <typename T>
 __global__ __launch_bounds__(64) void chol_leaf_kernel(T *A, int lda, int k0, int kb, int *info, T *Dinv, int ldd) {
     if (*info) return;
+    __shared__ T Ls[NB][NB];
     const int r = threadIdx.x;
     T a[NB];
 #pragma unroll
@@ -549,24 +548,33 @@ __global__ __launch_bounds__(64) void chol_leaf_kernel(T *A, int lda, int k0, in
     }
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
-        const T p = lane_bcast(a[j], j);
+        Ls[j][r] = a[j];
+        __syncthreads();
+        const T p = Ls[j][j];
         if (!(p > (T) 0)) {
             if (r == 0) *info = k0 + j + 1;
             return;
         }
         const T t = a[j] / p;
 #pragma unroll
-        for (int c = j + 1; c < NB; ++c) a[c] -= t * lane_bcast(a[j], c);
+        for (int c = j + 1; c < NB; ++c) a[c] -= t * Ls[j][c];
         const T sq = sqrt(p);
         a[j] = (r == j) ? sq : a[j] / sq;
     }
-    T x[NB];
-    leaf_inverse<T>(a, r, x);
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NB; ++c) Ls[c][r] = a[c];
 #pragma unroll
     for (int c = 0; c < NB; ++c)
         if (r < kb && c < kb && r >= c) A[r + (long) c * lda] = a[c];
+    __syncthreads();
+    T x[NB];
+    leaf_inverse<T>(Ls, r, x);
+    __syncthreads();
 #pragma unroll
-    for (int k = 0; k < NB; ++k) Dinv[k + (long) r * ldd] = (k >= r) ? x[k] : (T) 0;
+    for (int k = 0; k < NB; ++k) Ls[r][k] = (k >= r) ? x[k] : (T) 0;
+    __syncthreads();
+    for (int c = 0; c < NB; ++c) Dinv[r + (long) c * ldd] = Ls[c][r];
 }
 
 // Right-looking blocked Cholesky.  Per block column k (width NB):
@@ -640,26 +648,30 @@ template bool potrf<float>(hipStream_t, int, float *, int, float *, size_t, int 
 // ---- triangular inverse ------------------------------------------------------
 
 // Inverse of each 64 x 64 diagonal block of the lower factor (the last one may be
-// smaller, padded with the identity): one wave per block, lane q loads row q.
+// smaller, padded with the identity): one wave per block, lane q loads row q and
+// stores it to LDS as column-major L for the broadcast reads of the substitution.
 template <typename T>
 __global__ __launch_bounds__(64) void trtri_diag_kernel(int n, const T *L, int ldl, T *Linv, int ldi) {
     const int o = blockIdx.x * NB;
     const int kb = (n - o < NB) ? n - o : NB;
     const int r = threadIdx.x;
-    T a[NB];
+    __shared__ T Ls[NB][NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
         T v = (r == c) ? (T) 1 : (T) 0;
         if (r < kb && c < kb && r >= c) v = L[(o + r) + (long) (o + c) * ldl];
-        a[c] = v;
+        Ls[c][r] = v;
     }
+    __syncthreads();
     T x[NB];
-    leaf_inverse<T>(a, r, x);
-    if (r < kb) {
+    leaf_inverse<T>(Ls, r, x);
+    __syncthreads();
+    // column r of the inverse back into LDS, then stored row by row (lane q writes row q)
 #pragma unroll
-        for (int k = 0; k < NB; ++k)
-            if (k < kb) Linv[(o + k) + (long) (o + r) * ldi] = (k >= r) ? x[k] : (T) 0;
-    }
+    for (int k = 0; k < NB; ++k) Ls[r][k] = (k >= r) ? x[k] : (T) 0;
+    __syncthreads();
+    for (int c = 0; c < kb; ++c)
+        if (r < kb) Linv[(o + r) + (long) (o + c) * ldi] = Ls[c][r];
 }
 
 template <typename T>
