@@ -523,58 +523,132 @@ __device__ inline void leaf_inverse(const T (&Ls)[NB][NB], int r, T (&x)[NB]) {
     }
 }
 
-// One wave factors a kb x kb diagonal block (kb <= 64) and inverts its factor.
-// Lane r holds row r in registers; at step j every lane stores its entry of column j
-// in LDS and reads the column back as broadcasts:
+// Four waves factor a kb x kb diagonal block (kb <= 64) and invert its factor.
+// Thread (w, r) holds row r, columns 16 w .. 16 w + 15, in registers; every column
+// is published in LDS once it is final for the step that needs it:
 //   step j: p = a(j, j); row r: t = a(r, j) / p; a(r, c) -= t a(c, j) for c > j;
-//           then column j is scaled by 1 / sqrt(p) and a(j, j) = sqrt(p).
-// Entries above the diagonal are scratch.  The inverse X = L^-1 follows by forward
-// substitution from L in LDS, lane c computing column c.  Rows and columns beyond kb
-// are the identity.  Writes L (lower) in place and X into Dinv (64 x 64, zeros
-// above); sets *info = k0 + j + 1 at the first non-positive pivot.  (The LDS
-// broadcast replaced lane-to-lane reads: one v_readlane pair per operand plus hazard
-// waits, three times the instructions; the arithmetic is the same.)
+//           column j is then scaled by 1 / sqrt(p), a(j, j) = sqrt(p).
+// The wave owning column j + 1 updates that column first and publishes it before its
+// other updates, and the scaling of each column waits until after the last step (no
+// step reads a scaled column), so only one division, one FMA and one LDS round trip
+// lie between consecutive steps.  Entries above the diagonal are scratch.
+// The inverse X = L^-1 follows by forward substitution, lane r computing column r of
+// X with wave w holding its rows 16 w .. 16 w + 15: after row j is published, the
+// owner of row j + 1 finishes that row (x(j+1) -= L(j+1, j) x(j), then the division
+// by L(j+1, j+1)) and publishes it, and every wave updates its other rows
+// k > j by x(k) -= L(k, j) x(j).  Every entry sees the same operations in the same
+// order as in a one-wave sweep.  Rows and columns beyond kb are the identity.  Writes
+// L (lower) in place and X into Dinv (64 x 64, zeros above); sets *info = k0 + j + 1
+// at the first non-positive pivot.
+constexpr int LEAF_W = 4, LEAF_CW = NB / LEAF_W;
+
 template <typename T>
-__global__ __launch_bounds__(64) void chol_leaf_kernel(T *A, int lda, int k0, int kb, int *info, T *Dinv, int ldd) {
+__global__ __launch_bounds__(256) void chol_leaf_kernel(T *A, int lda, int k0, int kb, int *info, T *Dinv, int ldd) {
     if (*info) return;
-    __shared__ T Ls[NB][NB];
-    const int r = threadIdx.x;
-    T a[NB];
+    __shared__ T Ls[NB][NB];      // Ls[c][q] = entry (q, c)
+    __shared__ T Xs[NB][NB + 1];  // Xs[k][c] = X(k, c)
+    const int r = threadIdx.x & 63, w = threadIdx.x >> 6, c0 = w * LEAF_CW;
+    T a[LEAF_CW], sq[LEAF_CW];
 #pragma unroll
-    for (int c = 0; c < NB; ++c) {
+    for (int i = 0; i < LEAF_CW; ++i) {
+        const int c = c0 + i;
         T v = (r == c) ? (T) 1 : (T) 0;
         if (r < kb && c < kb && r >= c) v = A[r + (long) c * lda];
-        a[c] = v;
+        a[i] = v;
+        sq[i] = (T) 1;
+    }
+    if (w == 0) Ls[0][r] = a[0];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const int wj = j / LEAF_CW, jj = j % LEAF_CW;
+        const int jn = (j + 1) % LEAF_CW;  // the next column's index in its owner wave
+        const bool last = j + 1 == NB;
+        __syncthreads();  // column j is published
+        const T p = Ls[j][j];
+        if (!(p > (T) 0)) {
+            if (threadIdx.x == 0) *info = k0 + j + 1;
+            return;
+        }
+        if (w == wj) {
+            const T t = a[jj] / p;
+            if (jj + 1 < LEAF_CW) {  // the next column is this wave's: update and publish it first
+                a[jj + 1] -= t * Ls[j][j + 1];
+                Ls[j + 1][r] = a[jj + 1];
+            }
+#pragma unroll
+            for (int i = jj + 2; i < LEAF_CW; ++i) a[i] -= t * Ls[j][c0 + i];
+            sq[jj] = sqrt(p);
+        } else if (w > wj) {
+            const T t = Ls[j][r] / p;
+            if (!last && jn == 0 && w == wj + 1) {  // this wave owns the next column
+                a[0] -= t * Ls[j][c0];
+                Ls[j + 1][r] = a[0];
+#pragma unroll
+                for (int i = 1; i < LEAF_CW; ++i) a[i] -= t * Ls[j][c0 + i];
+            } else {
+#pragma unroll
+                for (int i = 0; i < LEAF_CW; ++i) a[i] -= t * Ls[j][c0 + i];
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < LEAF_CW; ++i) a[i] = (r == c0 + i) ? sq[i] : a[i] / sq[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < LEAF_CW; ++i) {
+        const int c = c0 + i;
+        Ls[c][r] = a[i];
+        if (r < kb && c < kb && r >= c) A[r + (long) c * lda] = a[i];
+    }
+    __syncthreads();
+    // forward substitution: lane r builds column r of X, wave w its rows c0 .. c0 + 15
+    T dg[LEAF_CW], sub[LEAF_CW];  // L(k, k) and L(k, k - 1) of this wave's rows k
+#pragma unroll
+    for (int i = 0; i < LEAF_CW; ++i) {
+        dg[i] = Ls[c0 + i][c0 + i];
+        sub[i] = (c0 + i > 0) ? Ls[c0 + i - 1][c0 + i] : (T) 0;
+    }
+    int rr = r;
+    asm volatile("" : "+v"(rr));  // identity formed here, not merged with the padding above
+    T x[LEAF_CW];
+#pragma unroll
+    for (int i = 0; i < LEAF_CW; ++i) x[i] = (c0 + i == rr) ? (T) 1 : (T) 0;
+    if (w == 0) {
+        x[0] = x[0] / dg[0];
+        Xs[0][r] = x[0];
     }
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
-        Ls[j][r] = a[j];
-        __syncthreads();
-        const T p = Ls[j][j];
-        if (!(p > (T) 0)) {
-            if (r == 0) *info = k0 + j + 1;
-            return;
-        }
-        const T t = a[j] / p;
+        const int wj = j / LEAF_CW, jj = j % LEAF_CW;
+        const int jn = (j + 1) % LEAF_CW;
+        const bool last = j + 1 == NB;
+        __syncthreads();  // row j of X is published
+        if (w == wj) {
+            const T xj = x[jj];
+            if (jj + 1 < LEAF_CW) {  // finish and publish the next row first
+                x[jj + 1] -= sub[jj + 1] * xj;
+                x[jj + 1] = x[jj + 1] / dg[jj + 1];
+                Xs[j + 1][r] = x[jj + 1];
+            }
 #pragma unroll
-        for (int c = j + 1; c < NB; ++c) a[c] -= t * Ls[j][c];
-        const T sq = sqrt(p);
-        a[j] = (r == j) ? sq : a[j] / sq;
+            for (int i = jj + 2; i < LEAF_CW; ++i) x[i] -= Ls[j][c0 + i] * xj;
+        } else if (w > wj) {
+            const T xj = Xs[j][r];
+            if (!last && jn == 0 && w == wj + 1) {
+                x[0] -= sub[0] * xj;
+                x[0] = x[0] / dg[0];
+                Xs[j + 1][r] = x[0];
+#pragma unroll
+                for (int i = 1; i < LEAF_CW; ++i) x[i] -= Ls[j][c0 + i] * xj;
+            } else {
+#pragma unroll
+                for (int i = 0; i < LEAF_CW; ++i) x[i] -= Ls[j][c0 + i] * xj;
+            }
+        }
     }
     __syncthreads();
-#pragma unroll
-    for (int c = 0; c < NB; ++c) Ls[c][r] = a[c];
-#pragma unroll
-    for (int c = 0; c < NB; ++c)
-        if (r < kb && c < kb && r >= c) A[r + (long) c * lda] = a[c];
-    __syncthreads();
-    T x[NB];
-    leaf_inverse<T>(Ls, r, x);
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NB; ++k) Ls[r][k] = (k >= r) ? x[k] : (T) 0;
-    __syncthreads();
-    for (int c = 0; c < NB; ++c) Dinv[r + (long) c * ldd] = Ls[c][r];
+    // Dinv(q, c) = X(q, c) below the diagonal, stored row-coalesced: lane q writes row q
+    for (int c = c0; c < c0 + LEAF_CW; ++c) Dinv[r + (long) c * ldd] = (r >= c) ? Xs[r][c] : (T) 0;
 }
 
 // Right-looking blocked Cholesky.  Per block column k (width NB):
@@ -594,7 +668,7 @@ bool potrf(hipStream_t s, int n, T *A, int lda, T *work, size_t workLen, int *in
     auto at = [&](int i, int j) { return A + i + (long) j * lda; };
     auto leaf = [&](int k0) {
         const int kb = (n - k0 < NB) ? n - k0 : NB;
-        chol_leaf_kernel<T><<<1, 64, 0, s>>>(at(k0, k0), lda, k0, kb, info, work, NB);
+        chol_leaf_kernel<T><<<1, 256, 0, s>>>(at(k0, k0), lda, k0, kb, info, work, NB);
         if (k0 + kb < n)
             gemm<T>(s, false, true, false, n - k0 - kb, kb, kb, (T) 1, at(k0 + kb, k0), lda, 0, work, NB, 0, (T) 0,
                     at(k0 + kb, k0), lda, 0, 1);
