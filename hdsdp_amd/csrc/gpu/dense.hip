@@ -71,6 +71,48 @@ __device__ inline void tile_xy(int lin, int &bx, int &by, int tri = 0) {
     by = t - x * (x + 1) / 2;
 }
 
+// One k slab of a 64-wide tile into registers: thread t moves elements t + 256 u of
+// each operand's 64 x 16 slab (zero outside the matrix).
+template <typename T, bool TA, bool TB>
+__device__ inline void tile_fetch(const T *A, int lda, const T *B, int ldb, int M, int N, int K, int i0, int j0,
+                                  int k0, int tid, T (&ra)[4], T (&rb)[4]) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int e = tid + 256 * u;
+        int i, k;
+        if (!TA) { i = e & 63; k = e >> 6; } else { k = e & 15; i = e >> 4; }
+        const int gi = i0 + i, gk = k0 + k;
+        T v = 0;
+        if (gi < M && gk < K) v = TA ? A[gk + (long) gi * lda] : A[gi + (long) gk * lda];
+        ra[u] = v;
+        int j, kb;
+        if (!TB) { kb = e & 15; j = e >> 4; } else { j = e & 63; kb = e >> 6; }
+        const int gj = j0 + j, gkb = k0 + kb;
+        T w = 0;
+        if (gj < N && gkb < K) w = TB ? B[gj + (long) gkb * ldb] : B[gkb + (long) gj * ldb];
+        rb[u] = w;
+    }
+}
+
+template <typename T, bool TA, bool TB>
+__device__ inline void tile_stash(T (*As)[TILE + 1], T (*Bs)[TILE + 1], int tid, const T (&ra)[4],
+                                  const T (&rb)[4]) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int e = tid + 256 * u;
+        int i, k;
+        if (!TA) { i = e & 63; k = e >> 6; } else { k = e & 15; i = e >> 4; }
+        As[k][i] = ra[u];
+        int j, kb;
+        if (!TB) { kb = e & 15; j = e >> 4; } else { j = e & 63; kb = e >> 6; }
+        Bs[kb][j] = rb[u];
+    }
+}
+
+// 64 x 64 tiles: the C tile (beta != 0) is fetched before the k loop and the k slabs
+// are double-buffered (slab s + 1 loads in flight while slab s is multiplied), so a
+// short-k product such as a rank-64 update does not wait on memory once per slab and
+// again for C.
 template <typename T, bool TA, bool TB, bool LOWER>
 __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha, const T *A, int lda,
                                                    long sA, const T *B, int ldb, long sB, T beta,
@@ -85,46 +127,11 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha,
     A += (long) blockIdx.z * sA;
     B += (long) blockIdx.z * sB;
     C += (long) blockIdx.z * sC;
-    __shared__ T As[KSTEP][TILE + 1];
-    __shared__ T Bs[KSTEP][TILE + 1];
+    __shared__ T As[2][KSTEP][TILE + 1];
+    __shared__ T Bs[2][KSTEP][TILE + 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
     const int lr = lane & 15, lk = lane >> 4;
-    acc_t acc[2][2];
-    for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b) acc[a][b] = (acc_t){0, 0, 0, 0};
-
-    for (int k0 = kLo; k0 < kHi; k0 += KSTEP) {
-#pragma unroll
-        for (int e = tid; e < TILE * KSTEP; e += 256) {
-            int i, k;
-            if (!TA) { i = e & 63; k = e >> 6; } else { k = e & 15; i = e >> 4; }
-            int gi = i0 + i, gk = k0 + k;
-            T v = 0;
-            if (gi < M && gk < K) v = TA ? A[gk + (long) gi * lda] : A[gi + (long) gk * lda];
-            As[k][i] = v;
-        }
-#pragma unroll
-        for (int e = tid; e < TILE * KSTEP; e += 256) {
-            int j, k;
-            if (!TB) { k = e & 15; j = e >> 4; } else { j = e & 63; k = e >> 6; }
-            int gj = j0 + j, gk = k0 + k;
-            T v = 0;
-            if (gj < N && gk < K) v = TB ? B[gj + (long) gk * ldb] : B[gk + (long) gj * ldb];
-            Bs[k][j] = v;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < KSTEP; kk += 4) {
-            T a0 = As[kk + lk][wi + lr], a1 = As[kk + lk][wi + 16 + lr];
-            T b0 = Bs[kk + lk][wj + lr], b1 = Bs[kk + lk][wj + 16 + lr];
-            acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
-            acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
-            acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
-            acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
-        }
-        __syncthreads();
-    }
     T cin[2][2][4];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -137,6 +144,32 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha,
                 const bool ok = i < M && j < N && (!LOWER || i >= j);
                 cin[a][b][r] = (ok && beta != (T) 0) ? C[i + (long) j * ldc] : (T) 0;
             }
+    acc_t acc[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) acc[a][b] = (acc_t){0, 0, 0, 0};
+    const int nslab = kHi > kLo ? (kHi - kLo + KSTEP - 1) / KSTEP : 0;
+    if (nslab > 0) {
+        T ra[4], rb[4];
+        tile_fetch<T, TA, TB>(A, lda, B, ldb, M, N, K, i0, j0, kLo, tid, ra, rb);
+        tile_stash<T, TA, TB>(As[0], Bs[0], tid, ra, rb);
+        __syncthreads();
+        for (int sl = 0; sl < nslab; ++sl) {
+            const int cur = sl & 1;
+            const bool more = sl + 1 < nslab;
+            if (more) tile_fetch<T, TA, TB>(A, lda, B, ldb, M, N, K, i0, j0, kLo + (sl + 1) * KSTEP, tid, ra, rb);
+#pragma unroll
+            for (int kk = 0; kk < KSTEP; kk += 4) {
+                T a0 = As[cur][kk + lk][wi + lr], a1 = As[cur][kk + lk][wi + 16 + lr];
+                T b0 = Bs[cur][kk + lk][wj + lr], b1 = Bs[cur][kk + lk][wj + 16 + lr];
+                acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
+                acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
+                acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
+                acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
+            }
+            if (more) tile_stash<T, TA, TB>(As[cur ^ 1], Bs[cur ^ 1], tid, ra, rb);
+            __syncthreads();
+        }
+    }
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
