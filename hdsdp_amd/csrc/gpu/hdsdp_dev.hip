@@ -63,13 +63,14 @@ struct hdsdp_dev {
     double *coef_d, *vecA_d, *vecB_d, *sc_d;
     int *info_d;
     // pinned staging
-    double *hcoef, *hvecA, *hvecB, *hsc, *hvec_n, *hY;
+    double *hcoef, *hvecA, *hvecB, *hsc, *hvec_n, *hY, *hHist;
     int *hinfo;
     bool stagingBusy;
 
     // Lanczos (linalg/hdsdp_lanczos.c)
     int lzMax, lzComputed;
     double *V, *Hd, *lv, *lw, *lz1, *lz2, *lwarm, *ly, *mv1, *mv2;
+    double *lzHist;  // (alpha_k, ||w_k||) of every Lanczos step, read at the check steps
     std::vector<double> Hh;
 
     // Schur system
@@ -208,7 +209,13 @@ void lz_perturb(int n, double *vVec) {
     }
 }
 
-// HLanczosSolve (linalg/hdsdp_lanczos.c:161-292) with device mat-vecs
+// HLanczosSolve (linalg/hdsdp_lanczos.c:161-292) with device mat-vecs.  The host
+// needs a step's (alpha, ||w||) only for the tridiagonal at the check steps (every
+// `check` steps and at a breakdown), so the steps run back to back and their scalars
+// are read once per check window and processed step by step in the reference's order.
+// A breakdown or a converged check inside a window has let the device run the rest
+// of the window's steps; those only write Krylov columns beyond the ones the
+// estimate combines, so the result is that of the step-by-step loop.
 hdsdp_retcode lanczos(hdsdp_dev *d, int f, double *maxStep) {
     const int n = d->n, maxDim = d->lzMax, nH = maxDim + 1;
     std::vector<double> &H = d->Hh;
@@ -226,22 +233,28 @@ hdsdp_retcode lanczos(hdsdp_dev *d, int f, double *maxStep) {
     hdk::lz_start(d->st, n, d->lw, d->lv, d->V);
     const int check = std::min(maxDim / 5, 3);
     double Td[32], Te[32], evals[2], Y[2 * 32];
-    for (int k = 0; k < maxDim; ++k) {
+    int kRead = 0;  // first step whose scalars the host has not processed
+    bool done = false;
+    for (int k = 0; k < maxDim && !done; ++k) {
         lz_mvec(d, f, d->lv, d->lw);
-        hdk::lz_step(d->st, n, k, d->lw, d->V, d->Hd, nH, d->lv, d->sc_d);
-        DCALL(read_scalars(d));
-        double alpha = d->hsc[hdk::LZ_ALPHA], normPres = d->hsc[hdk::LZ_NORM];
-        H[k + k * nH] = alpha;
-        if (normPres > 0.0) H[(k + 1) + k * nH] = H[k + (k + 1) * nH] = normPres;
-        if ((k + 1) % check == 0 || k > maxDim - 1 || normPres == 0.0) {
-            const int kp1 = k + 1;
+        hdk::lz_step(d->st, n, k, d->lw, d->V, d->Hd, nH, d->lv, d->sc_d, d->lzHist);
+        if ((k + 1) % check != 0 && k < maxDim - 1) continue;
+        HIP_OK(hipMemcpyAsync(d->hHist + 2 * kRead, d->lzHist + 2 * kRead, sizeof(double) * 2 * (k + 1 - kRead),
+                              hipMemcpyDeviceToHost, d->st));
+        DCALL(sync(d));
+        for (int j = kRead; j <= k && !done; ++j) {
+            const double alpha = d->hHist[2 * j], normPres = d->hHist[2 * j + 1];
+            H[j + j * nH] = alpha;
+            if (normPres > 0.0) H[(j + 1) + j * nH] = H[j + (j + 1) * nH] = normPres;
+            if (!((j + 1) % check == 0 || j > maxDim - 1 || normPres == 0.0)) continue;
+            const int kp1 = j + 1;
             for (int i = 0; i < kp1; ++i) {
                 Td[i] = H[i + nH * i];
                 if (i + 1 < kp1) Te[i] = H[(i + 1) + nH * i];
             }
             if (kp1 < 2 || hdsdp_eig_tridiag_top(kp1, Td, Te, 2, evals, Y) != 0) return HDSDP_RETCODE_FAILED;
-            double resi = fabs(H[kp1 + k * nH] * Y[kp1 + k]);
-            if (resi < 1e-04 || k >= maxDim - 1) {
+            double resi = fabs(H[kp1 + j * nH] * Y[kp1 + j]);
+            if (resi < 1e-04 || j >= maxDim - 1) {
                 double e1 = evals[1], e2 = evals[0];
                 memcpy(d->hY, Y, sizeof(double) * 2 * kp1);
                 HIP_OK(hipMemcpyAsync(d->ly, d->hY, sizeof(double) * 2 * kp1, hipMemcpyHostToDevice, d->st));
@@ -260,13 +273,14 @@ hdsdp_retcode lanczos(hdsdp_dev *d, int f, double *maxStep) {
                 gamma = std::min(r1, r1sq);
                 if (gamma < 1e-03 || gamma + e1 <= 0.5) {
                     *maxStep = (gamma + e1 <= 0.0) ? HDSDP_INFINITY : 1.0 / (gamma + e1);
-                    break;
+                    done = true;
                 } else {
                     if (normPres == 0.0) return HDSDP_RETCODE_FAILED;
                     *maxStep = 1.0 / (gamma + e1);
                 }
             }
         }
+        kRead = k + 1;
     }
     d->lzComputed += 1;
     return check_launch();
@@ -659,6 +673,8 @@ hdsdp_retcode hdsdp_dev_create(hdsdp_dev **pdev, int nRow, int nCol, int precisi
     d->lzMax = 30;
     d->lzComputed = 0;
     d->Hh.assign((size_t) (d->lzMax + 1) * (d->lzMax + 1), 0.0);
+    DCALL(halloc(&d->hHist, 2 * (d->lzMax + 1)));
+    DCALL(dalloc(d, &d->lzHist, 2 * (d->lzMax + 1)));
     DCALL(dalloc(d, &d->V, n * (d->lzMax + 1)));
     DCALL(dalloc(d, &d->Hd, (size_t) (d->lzMax + 1) * (d->lzMax + 1)));
     DCALL(dalloc(d, &d->lv, n));
@@ -986,7 +1002,7 @@ void hdsdp_dev_destroy(hdsdp_dev **pdev) {
     hdsdp_dev *d = *pdev;
     if (d->st) hipStreamSynchronize(d->st);
     for (void *p : d->allocs) hipFree(p);
-    double *hp[] = {d->hcoef, d->hvecA, d->hvecB, d->hsc, d->hvec_n, d->hY};
+    double *hp[] = {d->hcoef, d->hvecA, d->hvecB, d->hsc, d->hvec_n, d->hY, d->hHist};
     for (double *p : hp)
         if (p) hipHostFree(p);
     if (d->hinfo) hipHostFree(d->hinfo);

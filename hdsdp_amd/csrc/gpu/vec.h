@@ -30,7 +30,8 @@ void cg_residual(hipStream_t s, int m, const double *rhs, const double *Mx, doub
 // r = rhs - Mx and sc[CG_TRUE] = ||r||
 void cg_true_residual(hipStream_t s, int m, const double *rhs, const double *Mx, double *r, double *sc);
 
-void lz_step(hipStream_t s, int n, int k, double *w, double *V, double *H, int ldh, double *v, double *sc);
+void lz_step(hipStream_t s, int n, int k, double *w, double *V, double *H, int ldh, double *v, double *sc,
+             double *hist);
 void lz_start(hipStream_t s, int n, const double *x, double *v, double *V);
 void lz_comb(hipStream_t s, int n, int kp1, const double *V, const double *y, double *z);
 void axpy_norm(hipStream_t s, int n, double a, const double *x, double *y, double *sc, int slot);

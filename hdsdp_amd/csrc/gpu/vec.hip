@@ -139,7 +139,7 @@ void cg_residual(hipStream_t s, int m, const double *rhs, const double *Mx, doub
 //   norm = ||w||;  v = w / norm (or 0);  if norm > 0: V_{k+1} = v, H(k+1,k) = H(k,k+1) = norm
 // sc[LZ_ALPHA] = -alp, sc[LZ_NORM] = norm.
 __global__ __launch_bounds__(1024) void lz_step_kernel(int n, int k, double *w, double *V, double *H, int ldh, double *v,
-                                                       double *sc) {
+                                                       double *sc, double *hist) {
     __shared__ double red[17];
     const double *Vk = V + (long) k * n;
     if (k > 0) {
@@ -173,6 +173,8 @@ __global__ __launch_bounds__(1024) void lz_step_kernel(int n, int k, double *w, 
         }
         sc[LZ_ALPHA] = -alp;
         sc[LZ_NORM] = nrm;
+        hist[2 * k] = -alp;
+        hist[2 * k + 1] = nrm;
     }
 }
 
@@ -212,8 +214,9 @@ __global__ __launch_bounds__(1024) void axpy_norm_kernel(int n, double a, const 
     if (threadIdx.x == 0) sc[slot] = nrm;
 }
 
-void lz_step(hipStream_t s, int n, int k, double *w, double *V, double *H, int ldh, double *v, double *sc) {
-    lz_step_kernel<<<1, 1024, 0, s>>>(n, k, w, V, H, ldh, v, sc);
+void lz_step(hipStream_t s, int n, int k, double *w, double *V, double *H, int ldh, double *v, double *sc,
+             double *hist) {
+    lz_step_kernel<<<1, 1024, 0, s>>>(n, k, w, V, H, ldh, v, sc, hist);
 }
 void lz_start(hipStream_t s, int n, const double *x, double *v, double *V) {
     lz_start_kernel<<<1, 1024, 0, s>>>(n, x, v, V);
