@@ -902,6 +902,101 @@ template void coldot<double>(hipStream_t, int, int, int, double, const double *,
                              double *);
 template void coldot<float>(hipStream_t, int, int, int, float, const float *, int, const float *, float, float *);
 
+// ---- symmetric product from the lower triangle -----------------------------------
+
+// y = M x for symmetric M (column-major, ldm) reading only its lower triangle, in one
+// pass.  One wave per tile of SYMV_R rows x SYMV_C columns on or below the diagonal;
+// lane l owns rows r0 + l + 64 q (q < 4).  For each column j of the tile the wave
+// forms the column sum over its rows i >= j, sum_i M(i, j) x(i) (the (L^T x)(j)
+// share, L the lower triangle), and reduces it over the lanes into P1[R][j]; each
+// lane also accumulates the row sums over its columns j < i, sum_j M(i, j) x(j) (the
+// strictly-lower (L x)(i) share), stored as P2[C][i].  A second kernel adds the
+// partials in a fixed order, so the result is deterministic.  The triangle is read
+// once: half the bytes of the column-dot product over the full matrix.
+template <int RPT>
+__global__ __launch_bounds__(256) void symv_lower_tile_kernel(int m, const double *__restrict__ M, long ldm,
+                                                              const double *__restrict__ x, double *P1, double *P2) {
+    constexpr int TR = 64 * RPT;  // rows per tile
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int R = blockIdx.x * 4 + w, C = blockIdx.y;
+    const int r0 = R * TR, c0 = C * SYMV_C;
+    if (r0 >= m || r0 + TR - 1 < c0) return;  // no entry of this tile on or below the diagonal
+    double xi[RPT], acc2[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int i = r0 + lane + 64 * q;
+        xi[q] = (i < m) ? x[i] : 0.0;
+        acc2[q] = 0.0;
+    }
+    const int cEnd = min(min(c0 + SYMV_C, m), r0 + TR);
+    // two columns per pass, the next pair's loads issued before this pair's lane
+    // reductions (software pipelining: the loop is otherwise load-latency bound)
+    double v0[RPT], v1[RPT];
+    auto fetch = [&](int j) {
+        const double *col0 = M + (long) j * ldm, *col1 = col0 + ldm;
+        const bool two = j + 1 < cEnd;
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int i = r0 + lane + 64 * q;
+            v0[q] = (i < m && i >= j) ? col0[i] : 0.0;
+            v1[q] = (two && i < m && i >= j + 1) ? col1[i] : 0.0;
+        }
+    };
+    fetch(c0);
+    for (int j = c0; j < cEnd; j += 2) {
+        const bool two = j + 1 < cEnd;
+        const double xj0 = x[j], xj1 = two ? x[j + 1] : 0.0;
+        double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int i = r0 + lane + 64 * q;
+            a0 += v0[q] * xi[q];
+            if (i > j) acc2[q] += v0[q] * xj0;
+            a1 += v1[q] * xi[q];
+            if (i > j + 1) acc2[q] += v1[q] * xj1;
+        }
+        if (j + 2 < cEnd) fetch(j + 2);
+        for (int off = 32; off > 0; off >>= 1) {
+            a0 += __shfl_down(a0, off);
+            a1 += __shfl_down(a1, off);
+        }
+        if (lane == 0) {
+            P1[(long) R * m + j] = a0;
+            if (two) P1[(long) R * m + j + 1] = a1;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int i = r0 + lane + 64 * q;
+        if (i < m) P2[(long) C * m + i] = acc2[q];
+    }
+}
+
+// y(i) = sum over the row chunks R >= i / SYMV_R of P1[R][i] + sum over the column
+// blocks C <= (row chunk start of i + SYMV_R - 1) / SYMV_C of P2[C][i], in that order
+__global__ void symv_lower_reduce_kernel(int m, int nR, int nC, const double *P1, const double *P2, double *y) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    double s = 0.0;
+    for (int R = i / SYMV_R; R < nR; ++R) s += P1[(long) R * m + i];
+    const int cLast = min(nC - 1, ((i / SYMV_R) * SYMV_R + SYMV_R - 1) / SYMV_C);
+    for (int C = 0; C <= cLast; ++C) s += P2[(long) C * m + i];
+    y[i] = s;
+}
+
+size_t symv_lower_work(int m) {
+    const size_t nR = (m + SYMV_R - 1) / SYMV_R, nC = (m + SYMV_C - 1) / SYMV_C;
+    return (nR + nC) * (size_t) m;
+}
+
+void symv_lower(hipStream_t s, int m, const double *M, long ldm, const double *x, double *y, double *work) {
+    if (m <= 0) return;
+    const int nR = (m + SYMV_R - 1) / SYMV_R, nC = (m + SYMV_C - 1) / SYMV_C;
+    double *P1 = work, *P2 = work + (size_t) nR * m;
+    symv_lower_tile_kernel<SYMV_R / 64><<<dim3((nR + 3) / 4, nC), 256, 0, s>>>(m, M, ldm, x, P1, P2);
+    symv_lower_reduce_kernel<<<(m + 255) / 256, 256, 0, s>>>(m, nR, nC, P1, P2, y);
+}
+
 // ---- layout helpers --------------------------------------------------------------
 
 template <typename T>

@@ -82,6 +82,12 @@ void trtri(hipStream_t s, int n, const T *L, int ldl, T *Linv, int ldi, T *work)
 template <typename T>
 void coldot(hipStream_t s, int m, int ncols, int tri, T alpha, const T *A, int lda, const T *x, T beta, T *y);
 
+// y = M x for symmetric M read only in its lower triangle (one pass, tile partials
+// added in a fixed order); work: symv_lower_work(m) doubles.
+constexpr int SYMV_R = 512, SYMV_C = 128;
+size_t symv_lower_work(int m);
+void symv_lower(hipStream_t s, int m, const double *M, long ldm, const double *x, double *y, double *work);
+
 // A[upper] = A[lower]^T for the n x n matrix.
 template <typename T>
 void mirror_lower(hipStream_t s, int n, T *A, int lda);

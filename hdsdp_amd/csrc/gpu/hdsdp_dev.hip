@@ -71,6 +71,7 @@ struct hdsdp_dev {
     int lzMax, lzComputed;
     double *V, *Hd, *lv, *lw, *lz1, *lz2, *lwarm, *ly, *mv1, *mv2;
     double *lzHist;  // (alpha_k, ||w_k||) of every Lanczos step, read at the check steps
+    double *symvW;   // tile partials of the lower-triangle product with M (large m), else null
     std::vector<double> Hh;
 
     // Schur system
@@ -387,8 +388,12 @@ void apply_precond(hdsdp_dev *d, const double *r, double *z, double *p) {
     if (p) hdk::cg_jacobi(d->st, m, z, nullptr, p, nullptr);
 }
 
+// y = M x.  Large M (m >= 16384 by default, HDSDP_SYMV_LOWER_MIN overrides) is read only
+// in its lower triangle, half the bytes of the column-dot product over the full matrix;
+// smaller M keep the column-dot product and its summation order.
 void symv_M(hdsdp_dev *d, const double *x, double *y) {
-    hdk::coldot<double>(d->st, d->m, d->m, 0, 1.0, d->M, d->m, x, 0.0, y);
+    if (d->symvW) hdk::symv_lower(d->st, d->m, d->M, d->m, x, y, d->symvW);
+    else hdk::coldot<double>(d->st, d->m, d->m, 0, 1.0, d->M, d->m, x, 0.0, y);
 }
 
 // y = M x in FP64 without M: (M x)_i = <A_i, W (sum_j x_j A_j) W> for the S^-1 = W of
@@ -686,6 +691,12 @@ hdsdp_retcode hdsdp_dev_create(hdsdp_dev **pdev, int nRow, int nCol, int precisi
     DCALL(dalloc(d, &d->mv1, n));
     DCALL(dalloc(d, &d->mv2, n));
     DCALL(dalloc(d, &d->M, m * m));
+    {
+        long lowerMin = 16384;
+        if (const char *v = getenv("HDSDP_SYMV_LOWER_MIN")) lowerMin = atol(v);
+        d->symvW = nullptr;
+        if (lowerMin > 0 && (long) m >= lowerMin) DCALL(dalloc(d, &d->symvW, hdk::symv_lower_work((int) m)));
+    }
     DCALL(dalloc(d, &d->jac, m));
     DCALL(dalloc(d, &d->cr, m));
     DCALL(dalloc(d, &d->crn, m));

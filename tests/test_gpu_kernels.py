@@ -218,3 +218,36 @@ def test_rank_one_buffer_split_k(gpu_lib):
         g.close()
     Sref = dt.dense_buffer(coeffs, 1.0, -1.0, y, 0.7)
     assert rel(S, Sref) < 1e-11
+
+
+@pytest.mark.parametrize("m", [150, 700])
+def test_cg_solve_lower_triangle_product(gpu_lib, oracle, m, monkeypatch):
+    """CG on the one-pass product with M that reads only its lower triangle (the path
+    of m >= 16384, forced here for every size): 256-row by 128-column tiles, ragged at
+    both edges at m = 700, their partials added in a fixed order.  The solution against
+    numpy's solve of the same M and against the oracle's CG, within 1e-7."""
+    monkeypatch.setenv("HDSDP_SYMV_LOWER_MIN", "1")
+    rng = np.random.default_rng(13)
+    coeffs = dt.rand_coeffs(rng, 50, m, [dt.SPARSE, dt.SPR1, dt.DSR1])
+    y, eye = dt.interior_point(coeffs, rng)
+    diag = rng.uniform(0.1, 1.0, m)
+    rhs = rng.standard_normal(m)
+    g, o = dt.Dev(gpu_lib, coeffs), dt.Dev(oracle, coeffs)
+    sols = []
+    try:
+        for d in (g, o):
+            d.update(1.0, -1.0, y, eye, 0)
+            assert d.factorize(0) == 1
+            d.schur(0, 0.0)
+            d.add_diag(diag)
+            d.regularize(1e-6)
+            d.kkt_params(5e-12, 1e-12)
+            assert d.kkt_factorize() == 0
+            sols.append(d.kkt_solve(rhs))
+        M = g.get_schur()
+    finally:
+        g.close()
+        o.close()
+    x = np.linalg.solve(M, rhs)
+    assert rel(sols[0], x) < 1e-7
+    assert rel(sols[0], sols[1]) < 1e-7

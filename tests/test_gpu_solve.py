@@ -98,3 +98,20 @@ def test_setup_then_optimize_is_optimize(gpu_lib):
         r1 = sv.optimize()
     assert (r1.status, r1.iters) == (r0.status, r0.iters)
     assert np.array_equal(r1.y, r0.y)
+
+
+@pytest.mark.parametrize("make", [lambda: I.lovasz_theta(50, 0.3, 21), lambda: I.rank_one(60, 500, 5, 45)],
+                         ids=["theta_n50", "rankone_n60_m500"])
+def test_lower_triangle_product_solve(gpu_lib, make, monkeypatch):
+    """A full solve with every product with the Schur matrix on the lower-triangle
+    path (forced for every m): optimal, objectives within 1e-8 of the column-dot run;
+    the different summation order may move the iteration count by one."""
+    inst = make()
+    r0 = run_solver(gpu_lib, inst, relopt=1e-6, absopt=1e30)
+    monkeypatch.setenv("HDSDP_SYMV_LOWER_MIN", "1")
+    r1 = run_solver(gpu_lib, inst, relopt=1e-6, absopt=1e30)
+    print(r0.iters, r1.iters, r0.dobj, r1.dobj, flush=True)
+    assert r0.status == r1.status == 3
+    assert abs(r1.iters - r0.iters) <= 1
+    assert abs(r1.dobj - r0.dobj) <= 1e-8 * max(1.0, abs(r0.dobj))
+    assert abs(r1.pobj - r0.pobj) <= 1e-8 * max(1.0, abs(r0.pobj))
