@@ -274,36 +274,49 @@ void r1_schur(hipStream_t s, const DevCoeffs &c, const double *H, double *K, lon
 // ---- dense coefficients through a Gram matrix -------------------------------------------
 
 // out[b] = full n x n matrix of dense slot slot0 + b (unpacked from the lower packed column)
-__global__ void unpack_dense_kernel(int n, const double *dVal, int slot0, double *out) {
+// Full n x n copies of the packed dense coefficients slot0 .. slot0 + cnt - 1, one
+// workgroup per 64 x 64 lower tile: the tile is read from the packed columns (lane =
+// row, coalesced), written to its place and, through LDS, to its mirror above the
+// diagonal (lane = row of the mirror, coalesced).  Unpacking element by element read
+// the upper half with a stride of a packed column per lane.
+template <typename T>
+__global__ __launch_bounds__(256) void unpack_dense_kernel(int n, const double *dVal, int slot0, T *out) {
+    __shared__ T tile[64][65];  // tile[c][r] = A(i0 + r, j0 + c)
     const long npk = (long) n * (n + 1) / 2;
     const double *v = dVal + (long) (slot0 + blockIdx.y) * npk;
-    double *o = out + (long) blockIdx.y * n * n;
-    for (long e = blockIdx.x * (long) blockDim.x + threadIdx.x; e < (long) n * n; e += (long) gridDim.x * blockDim.x) {
-        long i = e % n, j = e / n;
-        long r = i > j ? i : j, q = i > j ? j : i;
-        o[e] = v[q * n - q * (q + 1) / 2 + r];
+    T *o = out + (long) blockIdx.y * n * n;
+    const int t = blockIdx.x;
+    int bi = (int) ((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
+    while (bi * (bi + 1) / 2 > t) --bi;
+    const int bj = t - bi * (bi + 1) / 2;
+    const int i0 = bi * 64, j0 = bj * 64, tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int c = ty; c < 64; c += 4) {
+        const long i = i0 + tx, j = j0 + c;
+        if (i < n && j < n && i >= j) {
+            const T val = (T) v[j * n - j * (j + 1) / 2 + i];
+            o[i + j * n] = val;
+            tile[c][tx] = val;
+        }
     }
-}
-
-__global__ void unpack_dense_f_kernel(int n, const double *dVal, int slot0, float *out) {
-    const long npk = (long) n * (n + 1) / 2;
-    const double *v = dVal + (long) (slot0 + blockIdx.y) * npk;
-    float *o = out + (long) blockIdx.y * n * n;
-    for (long e = blockIdx.x * (long) blockDim.x + threadIdx.x; e < (long) n * n; e += (long) gridDim.x * blockDim.x) {
-        long i = e % n, j = e / n;
-        long r = i > j ? i : j, q = i > j ? j : i;
-        o[e] = (float) v[q * n - q * (q + 1) / 2 + r];
+    __syncthreads();
+    for (int c = ty; c < 64; c += 4) {
+        // mirror entry (j0 + tx, i0 + c) = A(i0 + c, j0 + tx) = tile[tx][c], above the diagonal only
+        const long r = j0 + tx, q = i0 + c;
+        if (r < n && q < n && q > r) o[r + q * n] = tile[tx][c];
     }
 }
 
 void unpack_dense_f(hipStream_t s, const DevCoeffs &c, int slot0, int cnt, float *out) {
     if (cnt <= 0) return;
-    unpack_dense_f_kernel<<<dim3(256, cnt), 256, 0, s>>>(c.n, c.dVal, slot0, out);
+    const int nt = (c.n + 63) / 64;
+    unpack_dense_kernel<float><<<dim3(nt * (nt + 1) / 2, cnt), 256, 0, s>>>(c.n, c.dVal, slot0, out);
 }
 
 void unpack_dense(hipStream_t s, const DevCoeffs &c, int slot0, int cnt, double *out) {
     if (cnt <= 0) return;
-    unpack_dense_kernel<<<dim3(256, cnt), 256, 0, s>>>(c.n, c.dVal, slot0, out);
+    const int nt = (c.n + 63) / 64;
+    unpack_dense_kernel<double><<<dim3(nt * (nt + 1) / 2, cnt), 256, 0, s>>>(c.n, c.dVal, slot0, out);
 }
 
 // M(i, j) (lower) = sum over the partial Gram blocks, in block order; i, j index dense slots,
