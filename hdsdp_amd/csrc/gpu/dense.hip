@@ -182,6 +182,94 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha,
             }
 }
 
+// Two successive symmetric rank-K updates of a 64 x 64 tile grid,
+//   C = alpha A1 A1(rows j)^T + beta C,  then  C = alpha A2 A2(rows j)^T + beta C,
+// with C read and written once.  Each product runs the k slabs and MFMAs of
+// gemm_kernel (TA = false, TB = true, B = A) and each update rounds as gemm_kernel's
+// epilogue, so the result is that of two gemm_kernel launches with half the traffic
+// on C.  lin as in gemm_kernel (lower products with a square tile grid).
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void gemm_nt2_kernel(int M, int N, int K, T alpha, const T *A1, const T *A2, int lda,
+                                                       T beta, T *C, int ldc, int lin) {
+    typedef typename Mfma<T>::acc_t acc_t;
+    int bx, by;
+    tile_xy(lin, bx, by, 0);
+    const int i0 = bx * TILE, j0 = by * TILE;
+    if (LOWER && i0 + TILE - 1 < j0) return;
+    __shared__ T As[2][KSTEP][TILE + 1];
+    __shared__ T Bs[2][KSTEP][TILE + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
+    const int lr = lane & 15, lk = lane >> 4;
+    T cin[2][2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = i0 + wi + a * 16 + Mfma<T>::row(lk, r);
+                const int j = j0 + wj + b * 16 + lr;
+                const bool ok = i < M && j < N && (!LOWER || i >= j);
+                cin[a][b][r] = (ok && beta != (T) 0) ? C[i + (long) j * ldc] : (T) 0;
+            }
+    const int nslab = K > 0 ? (K + KSTEP - 1) / KSTEP : 0;
+    acc_t acc[2][2];
+    auto product = [&](const T *A) {
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b) acc[a][b] = (acc_t){0, 0, 0, 0};
+        if (nslab == 0) return;
+        T ra[4], rb[4];
+        tile_fetch<T, false, true>(A, lda, A, lda, M, N, K, i0, j0, 0, tid, ra, rb);
+        tile_stash<T, false, true>(As[0], Bs[0], tid, ra, rb);
+        __syncthreads();
+        for (int sl = 0; sl < nslab; ++sl) {
+            const int cur = sl & 1;
+            const bool more = sl + 1 < nslab;
+            if (more) tile_fetch<T, false, true>(A, lda, A, lda, M, N, K, i0, j0, (sl + 1) * KSTEP, tid, ra, rb);
+#pragma unroll
+            for (int kk = 0; kk < KSTEP; kk += 4) {
+                T a0 = As[cur][kk + lk][wi + lr], a1 = As[cur][kk + lk][wi + 16 + lr];
+                T b0 = Bs[cur][kk + lk][wj + lr], b1 = Bs[cur][kk + lk][wj + 16 + lr];
+                acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
+                acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
+                acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
+                acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
+            }
+            if (more) tile_stash<T, false, true>(As[cur ^ 1], Bs[cur ^ 1], tid, ra, rb);
+            __syncthreads();
+        }
+    };
+    product(A1);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) cin[a][b][r] = alpha * acc[a][b][r] + beta * cin[a][b][r];
+    product(A2);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = i0 + wi + a * 16 + Mfma<T>::row(lk, r);
+                const int j = j0 + wj + b * 16 + lr;
+                if (i < M && j < N && (!LOWER || i >= j)) C[i + (long) j * ldc] = alpha * acc[a][b][r] + beta * cin[a][b][r];
+            }
+}
+
+template <typename T>
+static void gemm_nt2(hipStream_t s, int M, int N, int K, T alpha, const T *A1, const T *A2, int lda, T beta, T *C,
+                     int ldc) {
+    if (M <= 0 || N <= 0) return;
+    const int mt = (M + TILE - 1) / TILE, nt = (N + TILE - 1) / TILE;
+    const int lin = (mt == nt) ? 1 : 0;
+    dim3 grid(lin ? mt * (mt + 1) / 2 : mt, lin ? 1 : nt, 1);
+    gemm_nt2_kernel<T, true><<<grid, 256, 0, s>>>(M, N, K, alpha, A1, A2, lda, beta, C, ldc, lin);
+}
+
 // 128 x 128 tiles for large products: four waves, each a 64 x 64 quadrant as 4 x 4
 // MFMA 16x16 accumulators; k slabs of 16 double-buffered in LDS so the global
 // loads of slab s + 1 are in flight while slab s is multiplied (one barrier per slab).
@@ -716,36 +804,45 @@ bool potrf(hipStream_t s, int n, T *A, int lda, T *work, size_t workLen, int *in
         }
         return true;
     }
+    // Block columns are taken in pairs (a, b = a + NB): the update of a's panel to the
+    // blocks from a + 2 NB on waits for b's panel, and then both updates run as one pass
+    // over the trailing matrix (gemm_nt2: the two updates in order, C read and written
+    // once; the trailing updates are bound by that traffic).  The next block column is
+    // always updated on s, ahead of its leaf; the rest on s2, overlapping the next leaves.
     int ring = 0;
-    bool pending = false;  // an update of s2 that the next block column must wait for
+    bool s2dirty = false;  // s2 holds an update that s has not yet waited for
     hipEvent_t evB = nullptr;
+    int kDef = -1;  // first panel of an open pair
     leaf(0);
     for (int k0 = 0;; k0 += NB) {
         const int k1 = k0 + NB;
         if (k1 >= n) break;
         const int nb1 = (n - k1 < NB) ? n - k1 : NB, k2 = k1 + nb1;
-        hipEvent_t evBprev = evB;
-        const bool waitPrev = pending;
-        if (k2 < n) {
-            // the rest of the trailing matrix on s2, as soon as this block column's panel is done
-            hipEvent_t evP = la->ev[ring++ % LA_EVENTS];
-            hipEventRecord(evP, s);
-            hipStreamWaitEvent(la->s2, evP, 0);
-            gemm<T>(la->s2, false, true, true, n - k2, n - k2, NB, (T) -1, at(k2, k0), lda, 0, at(k2, k0), lda, 0,
-                    (T) 1, at(k2, k2), lda, 0, 1);
-            evB = la->ev[ring++ % LA_EVENTS];
-            hipEventRecord(evB, la->s2);
-            pending = true;
+        if (kDef < 0) {
+            if (k2 < n) kDef = k0;  // this panel's update of blocks from k2 on waits for the next panel
+            if (s2dirty) {
+                hipStreamWaitEvent(s, evB, 0);  // the previous pair's update reached block k1
+                s2dirty = false;
+            }
+            gemm<T>(s, false, true, true, n - k1, nb1, NB, (T) -1, at(k1, k0), lda, 0, at(k1, k0), lda, 0, (T) 1,
+                    at(k1, k1), lda, 0, 1);
         } else {
-            pending = false;
+            if (k2 < n) {
+                hipEvent_t evP = la->ev[ring++ % LA_EVENTS];
+                hipEventRecord(evP, s);
+                hipStreamWaitEvent(la->s2, evP, 0);
+                gemm_nt2<T>(la->s2, n - k2, n - k2, NB, (T) -1, at(k2, kDef), at(k2, k0), lda, (T) 1, at(k2, k2),
+                            lda);
+                evB = la->ev[ring++ % LA_EVENTS];
+                hipEventRecord(evB, la->s2);
+                s2dirty = true;
+            }
+            gemm_nt2<T>(s, n - k1, nb1, NB, (T) -1, at(k1, kDef), at(k1, k0), lda, (T) 1, at(k1, k1), lda);
+            kDef = -1;
         }
-        // the next block column [k1, k2): its last earlier update ran on s2
-        if (waitPrev) hipStreamWaitEvent(s, evBprev, 0);
-        gemm<T>(s, false, true, true, n - k1, nb1, NB, (T) -1, at(k1, k0), lda, 0, at(k1, k0), lda, 0, (T) 1,
-                at(k1, k1), lda, 0, 1);
         leaf(k1);
     }
-    if (pending) hipStreamWaitEvent(s, evB, 0);
+    if (s2dirty) hipStreamWaitEvent(s, evB, 0);
     return true;
 }
 
