@@ -73,6 +73,7 @@ struct hdsdp_dev {
     double *lzHist;  // (alpha_k, ||w_k||) of every Lanczos step, read at the check steps
     double *symvW;   // tile partials of the lower-triangle product with M (large m), else null
     std::vector<double> Hh;
+    std::vector<double> lzStart, lzPert;  // Lanczos start vector and warm-start perturbation (fixed by n)
 
     // Schur system
     double *M, *Mchol, *MLi, *MLiT;
@@ -193,21 +194,34 @@ void lz_mvec(hdsdp_dev *d, int f, const double *in, double *out) {
     d->stats[3] += 1;
 }
 
-// glibc rand start vector and warm-start perturbation (linalg/hdsdp_lanczos.c:33-53)
-void lz_prepare(int n, double *vVec) {
-    srand(n);
-    for (int i = 0; i < n; ++i) {
-        srand(rand());
-        vVec[i] = sqrt(sqrt((rand() % 1627))) * (rand() % 2 - 0.5);
+// glibc rand start vector and warm-start perturbation (linalg/hdsdp_lanczos.c:33-53).
+// Both restart the generator from srand(n) and reseed it per entry, so they are fixed
+// by n: they are generated once per device (n reseeds of glibc's generator cost ~1.5 us
+// each, ~3 ms per Lanczos call at n = 2000) and reused.  The perturbation entries are
+// 1e-3 sqrt(sqrt(r)) times +-0.5 (an exact product), so adding the stored entry gives
+// the same sum as adding the expression.
+void lz_prepare(hdsdp_dev *d, int n, double *vVec) {
+    if ((int) d->lzStart.size() != n) {
+        d->lzStart.resize(n);
+        srand(n);
+        for (int i = 0; i < n; ++i) {
+            srand(rand());
+            d->lzStart[i] = sqrt(sqrt((rand() % 1627))) * (rand() % 2 - 0.5);
+        }
     }
+    memcpy(vVec, d->lzStart.data(), sizeof(double) * n);
 }
 
-void lz_perturb(int n, double *vVec) {
-    srand(n);
-    for (int i = 0; i < n; ++i) {
-        srand(rand());
-        vVec[i] += 1e-03 * sqrt(sqrt((rand() % 1627))) * (rand() % 2 - 0.5);
+void lz_perturb(hdsdp_dev *d, int n, double *vVec) {
+    if ((int) d->lzPert.size() != n) {
+        d->lzPert.resize(n);
+        srand(n);
+        for (int i = 0; i < n; ++i) {
+            srand(rand());
+            d->lzPert[i] = 1e-03 * sqrt(sqrt((rand() % 1627))) * (rand() % 2 - 0.5);
+        }
     }
+    for (int i = 0; i < n; ++i) vVec[i] += d->lzPert[i];
 }
 
 // HLanczosSolve (linalg/hdsdp_lanczos.c:161-292) with device mat-vecs.  The host
@@ -222,11 +236,11 @@ hdsdp_retcode lanczos(hdsdp_dev *d, int f, double *maxStep) {
     std::vector<double> &H = d->Hh;
     std::fill(H.begin(), H.end(), 0.0);
     if (d->lzComputed == 0) {
-        lz_prepare(n, d->hvec_n);
+        lz_prepare(d, n, d->hvec_n);
     } else {
         HIP_OK(hipMemcpyAsync(d->hvec_n, d->lwarm, n * sizeof(double), hipMemcpyDeviceToHost, d->st));
         DCALL(sync(d));
-        lz_perturb(n, d->hvec_n);
+        lz_perturb(d, n, d->hvec_n);
     }
     hdk::fill(d->st, d->V, (long) n * nH, 0.0);
     hdk::fill(d->st, d->Hd, (long) nH * nH, 0.0);
