@@ -1003,14 +1003,16 @@ template void coldot<float>(hipStream_t, int, int, int, float, const float *, in
 
 // y = M x for symmetric M (column-major, ldm) reading only its lower triangle, in one
 // pass.  One wave per tile of SYMV_R rows x SYMV_C columns on or below the diagonal;
-// lane l owns rows r0 + l + 64 q (q < 4).  For each column j of the tile the wave
+// lane l owns rows r0 + l + 64 q (q < RPT = SYMV_R / 64).  For each column j of the tile the wave
 // forms the column sum over its rows i >= j, sum_i M(i, j) x(i) (the (L^T x)(j)
 // share, L the lower triangle), and reduces it over the lanes into P1[R][j]; each
 // lane also accumulates the row sums over its columns j < i, sum_j M(i, j) x(j) (the
 // strictly-lower (L x)(i) share), stored as P2[C][i].  A second kernel adds the
 // partials in a fixed order, so the result is deterministic.  The triangle is read
-// once: half the bytes of the column-dot product over the full matrix.
-template <int RPT>
+// once: half the bytes of the column-dot product over the full matrix.  (Tile shape
+// measured at m = 20000: 512 x 256 with 4 columns per pass, 0.36 ms; 512 x 128 with 2,
+// 0.40 ms; wider or taller tiles leave too few waves.)
+template <int RPT, int CP>
 __global__ __launch_bounds__(256) void symv_lower_tile_kernel(int m, const double *__restrict__ M, long ldm,
                                                               const double *__restrict__ x, double *P1, double *P2) {
     constexpr int TR = 64 * RPT;  // rows per tile
@@ -1026,41 +1028,44 @@ __global__ __launch_bounds__(256) void symv_lower_tile_kernel(int m, const doubl
         acc2[q] = 0.0;
     }
     const int cEnd = min(min(c0 + SYMV_C, m), r0 + TR);
-    // two columns per pass, the next pair's loads issued before this pair's lane
+    // CP columns per pass, the next pass's loads issued before this pass's lane
     // reductions (software pipelining: the loop is otherwise load-latency bound)
-    double v0[RPT], v1[RPT];
+    double v[CP][RPT];
     auto fetch = [&](int j) {
-        const double *col0 = M + (long) j * ldm, *col1 = col0 + ldm;
-        const bool two = j + 1 < cEnd;
 #pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-            const int i = r0 + lane + 64 * q;
-            v0[q] = (i < m && i >= j) ? col0[i] : 0.0;
-            v1[q] = (two && i < m && i >= j + 1) ? col1[i] : 0.0;
+        for (int c = 0; c < CP; ++c) {
+            const double *col = M + (long) (j + c) * ldm;
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                const int i = r0 + lane + 64 * q;
+                v[c][q] = (j + c < cEnd && i < m && i >= j + c) ? col[i] : 0.0;
+            }
         }
     };
     fetch(c0);
-    for (int j = c0; j < cEnd; j += 2) {
-        const bool two = j + 1 < cEnd;
-        const double xj0 = x[j], xj1 = two ? x[j + 1] : 0.0;
-        double a0 = 0.0, a1 = 0.0;
+    for (int j = c0; j < cEnd; j += CP) {
+        double a[CP], xj[CP];
 #pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-            const int i = r0 + lane + 64 * q;
-            a0 += v0[q] * xi[q];
-            if (i > j) acc2[q] += v0[q] * xj0;
-            a1 += v1[q] * xi[q];
-            if (i > j + 1) acc2[q] += v1[q] * xj1;
+        for (int c = 0; c < CP; ++c) {
+            xj[c] = (j + c < cEnd) ? x[j + c] : 0.0;
+            a[c] = 0.0;
         }
-        if (j + 2 < cEnd) fetch(j + 2);
-        for (int off = 32; off > 0; off >>= 1) {
-            a0 += __shfl_down(a0, off);
-            a1 += __shfl_down(a1, off);
-        }
-        if (lane == 0) {
-            P1[(long) R * m + j] = a0;
-            if (two) P1[(long) R * m + j + 1] = a1;
-        }
+#pragma unroll
+        for (int c = 0; c < CP; ++c)
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                const int i = r0 + lane + 64 * q;
+                a[c] += v[c][q] * xi[q];
+                if (i > j + c) acc2[q] += v[c][q] * xj[c];
+            }
+        if (j + CP < cEnd) fetch(j + CP);
+        for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+            for (int c = 0; c < CP; ++c) a[c] += __shfl_down(a[c], off);
+        if (lane == 0)
+#pragma unroll
+            for (int c = 0; c < CP; ++c)
+                if (j + c < cEnd) P1[(long) R * m + j + c] = a[c];
     }
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
@@ -1090,7 +1095,7 @@ void symv_lower(hipStream_t s, int m, const double *M, long ldm, const double *x
     if (m <= 0) return;
     const int nR = (m + SYMV_R - 1) / SYMV_R, nC = (m + SYMV_C - 1) / SYMV_C;
     double *P1 = work, *P2 = work + (size_t) nR * m;
-    symv_lower_tile_kernel<SYMV_R / 64><<<dim3((nR + 3) / 4, nC), 256, 0, s>>>(m, M, ldm, x, P1, P2);
+    symv_lower_tile_kernel<SYMV_R / 64, 4><<<dim3((nR + 3) / 4, nC), 256, 0, s>>>(m, M, ldm, x, P1, P2);
     symv_lower_reduce_kernel<<<(m + 255) / 256, 256, 0, s>>>(m, nR, nC, P1, P2, y);
 }
 

@@ -84,7 +84,7 @@ void coldot(hipStream_t s, int m, int ncols, int tri, T alpha, const T *A, int l
 
 // y = M x for symmetric M read only in its lower triangle (one pass, tile partials
 // added in a fixed order); work: symv_lower_work(m) doubles.
-constexpr int SYMV_R = 512, SYMV_C = 128;
+constexpr int SYMV_R = 512, SYMV_C = 256;
 size_t symv_lower_work(int m);
 void symv_lower(hipStream_t s, int m, const double *M, long ldm, const double *x, double *y, double *work);
 
