@@ -251,3 +251,26 @@ def test_cg_solve_lower_triangle_product(gpu_lib, oracle, m, monkeypatch):
     x = np.linalg.solve(M, rhs)
     assert rel(sols[0], x) < 1e-7
     assert rel(sols[0], sols[1]) < 1e-7
+
+
+@pytest.mark.parametrize("n", [257, 300, 520, 777])
+def test_blocked_cholesky_look_ahead(gpu_lib, n):
+    """The blocked Cholesky with look-ahead (n > 4 panels of 64): trailing updates of
+    two panels per pass, odd and even panel counts, a ragged last panel.  log det S
+    against numpy's slogdet within 1e-10 relative, and a matrix shifted below its
+    smallest eigenvalue reported not positive definite."""
+    rng = np.random.default_rng(n)
+    coeffs = dt.rand_coeffs(rng, n, 12, [dt.DENSE])
+    y, eye = dt.interior_point(coeffs, rng)
+    g = dt.Dev(gpu_lib, coeffs)
+    try:
+        g.update(1.0, -1.0, y, eye, 0)
+        assert g.factorize(0) == 1
+        S = dt.dense_buffer(coeffs, 1.0, -1.0, y, eye)
+        ld = np.linalg.slogdet(S)[1]
+        assert abs(g.logdet(0) - ld) <= 1e-10 * max(1.0, abs(ld))
+        lam = np.linalg.eigvalsh(S).min()
+        g.update(1.0, -1.0, y, eye - lam - 1.0, 1)
+        assert g.factorize(1) == 0
+    finally:
+        g.close()
