@@ -835,6 +835,17 @@ hdsdp_retcode hdsdp_dev_load(hdsdp_dev *d, const hdsdp_coeff_view *cv) {
         }
     }
 
+    // single-entry small constraints (edges, diagonal entries): the LDS pair kernel
+    {
+        bool single = true;
+        int cnt = 0;
+        for (int k = 1; k <= m; ++k) {
+            if (smKind[k] == hdk::SM_R1 || (smKind[k] == hdk::SM_SP && smNnz[k] != 1)) single = false;
+            if (smKind[k] == hdk::SM_SP) ++cnt;
+        }
+        c.pairSingle = (single && cnt > 0 && n <= hdk::PAIR1_MAXN) ? 1 : 0;
+    }
+
     // Gram paths: one kind of constraint throughout (heavy sparse rows allowed)
     int nSmall = 0, nR1k = 0, nDnk = 0;
     for (int k = 1; k <= m; ++k) {

@@ -8,6 +8,9 @@ namespace hdk {
 constexpr int COEFF_ZERO = 0, COEFF_SPARSE = 1, COEFF_DENSE = 2, COEFF_SPR1 = 3, COEFF_DSR1 = 4;
 constexpr int SMALL_K = 4;                                   // entries per constraint in the pair kernel
 constexpr int SM_NONE = 0, SM_SP = 1, SM_R1 = 2, SM_HEAVY = 3;  // pair-kernel kinds
+constexpr int PAIR1_COLS = 8, PAIR1_MAXN = 480;                 // single-entry pair kernel: columns per
+                                                                 // workgroup, largest n (LDS: 16 n doubles,
+                                                                 // under 64 KB with the static arrays)
 
 // Device-resident coefficients of one SDP block (k = 0 is C, k = i + 1 is A_i).
 struct DevCoeffs {
@@ -36,6 +39,7 @@ struct DevCoeffs {
     double *entVal;
     int *smKind, *smNnz, *smRow, *smCol;  // (m + 1) x SMALL_K
     double *smVal;
+    int pairSingle;   // every small constraint is one sparse entry and n <= PAIR1_MAXN
 };
 
 // target = sum_k coef[k] A_k (coef[0] multiplies C) + eye I on the lower triangle.

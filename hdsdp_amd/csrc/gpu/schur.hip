@@ -467,8 +467,62 @@ __global__ __launch_bounds__(256) void schur_pair_kernel(int m, int n, DevCoeffs
     }
 }
 
+// The pair kernel when every small constraint is a single sparse entry (i, j) (graph
+// edges, diagonal entries) and n <= PAIR1_MAXN: a workgroup takes PAIR1_COLS columns q
+// of M, stages the two columns of W = S^-1 their entries (r_q, c_q) address in LDS,
+// and walks the rows p >= q: M(p, q) needs W(r_p, r_q), W(c_p, c_q), W(r_p, c_q) and
+// W(c_p, r_q), read from LDS instead of gathered from L2.  Each entry is formed by the
+// statements of pair_value for two single-entry constraints, in the same order.
+__global__ __launch_bounds__(256) void schur_pair_single_kernel(int m, int n, DevCoeffs c, const double *W, double *M,
+                                                                long ldm) {
+    extern __shared__ double wl[];  // [PAIR1_COLS][2][n]: W(:, r_q), W(:, c_q)
+    __shared__ int qr[PAIR1_COLS], qc[PAIR1_COLS], qok[PAIR1_COLS];
+    __shared__ double qv[PAIR1_COLS];
+    const int j0 = blockIdx.x * PAIR1_COLS, tid = threadIdx.x;
+    if (tid < PAIR1_COLS) {
+        const int j = j0 + tid, k = j + 1;
+        const bool ok = j < m && c.smKind[k] == SM_SP;
+        qok[tid] = ok;
+        qr[tid] = ok ? c.smRow[(long) k * SMALL_K] : 0;
+        qc[tid] = ok ? c.smCol[(long) k * SMALL_K] : 0;
+        qv[tid] = ok ? c.smVal[(long) k * SMALL_K] : 0.0;
+    }
+    __syncthreads();
+    for (int e = tid; e < PAIR1_COLS * 2 * n; e += blockDim.x) {
+        const int q = e / (2 * n), h = (e / n) & 1, row = e % n;
+        const int col = h ? qc[q] : qr[q];
+        wl[e] = qok[q] ? W[row + (long) col * n] : 0.0;
+    }
+    __syncthreads();
+    for (int i = j0 + tid; i < m; i += blockDim.x) {
+        const int k = i + 1;
+        if (c.smKind[k] != SM_SP) continue;
+        const int ar = c.smRow[(long) k * SMALL_K], ac = c.smCol[(long) k * SMALL_K];
+        const double sv = c.smVal[(long) k * SMALL_K];
+#pragma unroll
+        for (int q = 0; q < PAIR1_COLS; ++q) {
+            const int j = j0 + q;
+            if (!qok[q] || j > i) continue;
+            const double *wr = wl + (long) (2 * q) * n, *wc = wr + n;  // W(:, br), W(:, bc)
+            const int br = qr[q], bc = qc[q];
+            const double ov = qv[q];
+            double res = 0.0;
+            double tb = 0.0;
+            tb += ov * wr[ar] * wc[ac];
+            if (br != bc) tb += ov * wc[ar] * wr[ac];
+            res += (ar == ac) ? 0.5 * sv * tb : sv * tb;
+            M[i + (long) j * ldm] = 2.0 * res;
+        }
+    }
+}
+
 void schur_pairs(hipStream_t s, const DevCoeffs &c, const double *W, double *M, long ldm) {
     const int m = c.m;
+    if (c.pairSingle) {
+        const size_t lds = sizeof(double) * PAIR1_COLS * 2 * (size_t) c.n;
+        schur_pair_single_kernel<<<(m + PAIR1_COLS - 1) / PAIR1_COLS, 256, lds, s>>>(m, c.n, c, W, M, ldm);
+        return;
+    }
     long nt = (m + 63) / 64;
     long tiles = nt * (nt + 1) / 2;
     schur_pair_kernel<<<tiles, 256, 0, s>>>(m, c.n, c, W, M, ldm);
