@@ -9,7 +9,8 @@ seconds per IPM iteration over the timed steps.
 The JSON line also carries:
   * ``parity``: status, iterations and the error of y*, pObj and dObj against
     the reference HDSDP's FP64 result on the same instance (stored fixture);
-  * ``roofline``: the Schur pair-gather kernel (one launch per Schur build),
+  * ``roofline``: the Schur pair kernel (one launch per Schur build; the LDS
+    variant for single-entry constraints, as on C2),
     HBM-bound: algorithmic bytes = 8 per Schur entry written (lower triangle
     of M over the sparse constraints) + the n x n FP64 S^-1 read once,
     divided by its average launch time from HIP events on the library's
@@ -209,6 +210,10 @@ def main():
     else:
         n_small = 0
     pair_bytes = 8.0 * n_small * (n_small + 1) / 2 + 8.0 * n * n
+    # the library's kernel for this step: the LDS variant when every small constraint is
+    # a single entry and n <= 480, else the gather kernel
+    single = inst.factor is None and n_small > 0 and n <= 480 and bool(np.all(col_nnz[col_nnz <= 4] == 1))
+    pair_kernel = "schur_pair_single_kernel" if single else "schur_pair_kernel"
     pair_ms, pair_cnt = prof["pairs"]
     pair_avg_s = pair_ms / 1e3 / max(pair_cnt, 1)
     achieved = pair_bytes / pair_avg_s / 1e9 if pair_cnt else 0.0
@@ -250,7 +255,7 @@ def main():
         "parity": parity(results[-1], fixture if args.maxiter is None else None),
         "roofline": {
             "bound": "hbm",
-            "kernel": "schur_pair_kernel",
+            "kernel": pair_kernel,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
