@@ -115,6 +115,8 @@ def rand_coeffs(rng, n, m, kinds):
         k = kinds[i % len(kinds)]
         if k == SPARSE:
             out.append(rand_sparse(rng, n, int(rng.integers(1, 4))))
+        elif k == "SPARSE_ONE":
+            out.append(rand_sparse(rng, n, 1))
         elif k == "SPARSE_BIG":
             out.append(rand_sparse(rng, n, int(rng.integers(6, 3 * n))))
         elif k == DENSE:

@@ -17,6 +17,11 @@ CASES = {
     "dense": (30, 40, [dt.DENSE]),
     "dsr1": (35, 50, [dt.DSR1]),
     "all_kinds_blocks": (130, 150, [dt.SPARSE, dt.SPR1, dt.DENSE, dt.DSR1, "SPARSE_BIG", dt.ZERO]),
+    # single-entry constraints (edges, diagonal entries): the LDS pair kernel, with a
+    # ragged last block of Schur columns (m not a multiple of 8) and, in the second
+    # case, heavy rows and empty constraints among them
+    "single_entry": (45, 121, ["SPARSE_ONE"]),
+    "single_entry_heavy": (50, 150, ["SPARSE_ONE", "SPARSE_ONE", "SPARSE_BIG", dt.ZERO]),
 }
 
 
