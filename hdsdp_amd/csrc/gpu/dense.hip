@@ -661,8 +661,8 @@ __device__ inline void leaf_inverse(const T (&Ls)[NB][NB], int r, T (&x)[NB]) {
 // order as in a one-wave sweep.  Rows and columns beyond kb are the identity.  Writes
 // L (lower) in place and X into Dinv (64 x 64, zeros above); sets *info = k0 + j + 1
 // at the first non-positive pivot.  Eight waves of eight columns take 27 us for a
-// 64 x 64 block in FP64 against 41 us for four waves of sixteen: each step between
-// barriers is half as long (two waves per SIMD).
+// 64 x 64 block in FP64 against 41 us for four waves of sixteen (each step between
+// barriers is half as long, two waves per SIMD) and 28 us for sixteen waves of four.
 constexpr int LEAF_W = 8, LEAF_CW = NB / LEAF_W, LEAF_THREADS = LEAF_W * 64;
 
 template <typename T>
