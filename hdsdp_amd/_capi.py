@@ -40,6 +40,12 @@ _API = {
     "HUserDataChooseCone": (C.c_int, [P_VOID]),
     "HUserDataClear": (None, [P_VOID]),
     "HUserDataDestroy": (None, [C.POINTER(P_VOID)]),
+    # SDPA input (include/hdsdp_file_io.h)
+    "HReadSDPA": (C.c_int, [C.c_char_p, P_INT, P_INT, C.POINTER(P_INT), C.POINTER(P_DBL),
+                            C.POINTER(C.POINTER(P_INT)), C.POINTER(C.POINTER(P_INT)), C.POINTER(C.POINTER(P_DBL)),
+                            P_INT, P_INT, C.POINTER(P_INT), C.POINTER(P_INT), C.POINTER(P_DBL), P_INT]),
+    "HFreeSDPA": (None, [C.c_int, P_INT, P_DBL, C.POINTER(P_INT), C.POINTER(P_INT), C.POINTER(P_DBL), P_INT, P_INT,
+                         P_DBL]),
     # device ABI
     "hdsdp_dev_create": (C.c_int, [C.POINTER(P_VOID), C.c_int, C.c_int, C.c_int]),
     "hdsdp_dev_load": (C.c_int, [P_VOID, P_VOID]),
@@ -70,7 +76,7 @@ _API = {
 }
 
 # Device-time profile slots (HDSDP_PROF_* in include/hdsdp_gpu.h)
-PROF_SLOTS = ("schur", "pairs", "mfactor", "cg", "sfactor", "lanczos", "buffer", "primal")
+PROF_SLOTS = ("schur", "pairs", "mfactor", "cg", "sfactor", "lanczos", "buffer", "primal", "gram")
 
 # Entry points every build of the library exports (include/hdsdp.h, include/hdsdp_gpu.h).
 EXPORTED = tuple(_API)

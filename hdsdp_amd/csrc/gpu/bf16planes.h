@@ -1,0 +1,38 @@
+// bf16planes.h -- products on the bf16 matrix cores from operands stored as three bf16
+// planes (x = x0 + x1 + x2, FP32-equivalent), and the kernels that write such planes.
+#pragma once
+
+#include <hip/hip_bf16.h>
+
+#include "dev_common.h"
+#include "schur.h"
+
+namespace hdk {
+
+constexpr int BF16P_OUT_F32 = 0, BF16P_OUT_F64 = 1, BF16P_OUT_PLANES = 2;
+// padded leading dimension of a plane whose columns hold n entries (16-byte vectors)
+inline long plane_ld(long n) { return (n + 7) / 8 * 8; }
+
+// C = alpha X^T Y + beta C, X: K x M and Y: K x N column-major bf16 planes (plane t at
+// X + t psX, column stride ldx, batch stride sX), six bf16 products, FP32 accumulation
+// flushed into FP64 every 512 k.  K, ldx, ldy, psX, psY, sX, sY and the operand
+// addresses are multiples of 8 elements; rows K_true..K of the planes are zero.
+// outKind: BF16P_OUT_F32 / F64 (C of that type, ld ldc) or BF16P_OUT_PLANES (three bf16
+// planes of alpha X^T Y at C + t psC; beta is ignored).  lower restricts C to i >= j;
+// tri = 1 when Y(k, j) = 0 for k > j, 2 when X(k, i) = 0 for k > i (those k skipped).
+void gemm_bf16p(hipStream_t s, bool lower, int M, int N, int K, double alpha, const __bf16 *X, int ldx, long psX,
+                long sX, const __bf16 *Y, int ldy, long psY, long sY, double beta, void *C, int outKind, int ldc,
+                long psC, long sC, int batch = 1, int tri = 0);
+// planes (ld, plane stride ps) of a rows x cols matrix, rows..ld-1 zero
+void split_matrix(hipStream_t s, int rows, int cols, const double *A, long lda, __bf16 *P, int ld, long ps);
+void split_matrix_f(hipStream_t s, int rows, int cols, const float *A, long lda, __bf16 *P, int ld, long ps);
+// full n x n planes (ld ldn, plane stride ps, matrix stride stride) of dense slots
+// slot0 .. slot0 + cnt - 1; the padding rows are left as they are (zero)
+void unpack_dense_split(hipStream_t s, const DevCoeffs &c, int slot0, int cnt, __bf16 *out, int ldn, long ps,
+                        long stride);
+// T_p (cnt FP32 matrices n x n, lower valid) -> planes of the strictly lower part packed
+// by columns (column p at off + p ldo) and of the diagonal (dg + p ldd)
+void pack_lower_split(hipStream_t s, int n, int cnt, const float *Tin, long strideIn, __bf16 *off, long ldo, long pso,
+                      __bf16 *dg, int ldd, long psd);
+
+}  // namespace hdk

@@ -1,0 +1,374 @@
+// bf16planes.hip -- FP32-equivalent products on the bf16 matrix cores of gfx950 from
+// operands stored split.
+//
+// An FP32 (or FP64, rounded to FP32 first) value x is held as three bf16 planes,
+//   x = x0 + x1 + x2,  x0 = bf16(x), x1 = bf16(x - x0), x2 = bf16(x - x0 - x1),
+// exact for every FP32 x in the normal range (24 significant bits in three terms of 8).
+// C = alpha X^T Y + beta C is formed from the six bf16 products whose weight is at least
+// 2^-16 of the leading one: x0 y0 + (x1 y0 + x0 y1) + (x1 y1 + x0 y2 + x2 y0).  Every bf16 x
+// bf16 product is exact in FP32.  v_mfma_f32_16x16x32_bf16 accumulates the leading product
+// and the five correction products in two FP32 accumulators (the corrections are ~2^-8 of
+// the leading sum and would lose their low bits against it), and every PFLUSH k slabs both
+// are added into FP64 accumulators, so long sums (K = n^2 / 2) do not build up FP32
+// rounding.
+//
+// The split is done once per operand, by the kernel that produces it, instead of in the
+// product's inner loop: the loop only moves bf16 planes global -> registers -> LDS and
+// issues MFMAs.
+//
+// Layout: plane t of a K x M operand X at X + t * ps, column i (k-contiguous) at X + i * ld;
+// ld and every column offset are multiples of 8 (16-byte vector loads), and the k rows
+// from the true length up to ld are zero, so products run over whole 8-element groups.
+//
+// Tiles: 128 x 128 per workgroup of eight waves (4 x 2), each wave 32 x 64 (2 x 4 MFMA
+// 16 x 16 blocks); k slabs of 32 double-buffered in LDS (rows padded to 40 bf16 so that a
+// lane's 8 consecutive k are one aligned 16-byte read).  MFMA operand layout (gfx950):
+// A(i, k) for i = lane % 16 and k = 8 (lane / 16) + t in element t; B(k, j) likewise with
+// j = lane % 16; D(i, j) at lane j + 16 (i / 4), element i % 4.
+#include "dev_common.h"
+#include "bf16planes.h"
+
+#include <type_traits>
+
+namespace hdk {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int PT = 128;     // output tile
+constexpr int PK = 32;      // k slab = one MFMA k
+constexpr int PPAD = 8;     // row padding (bf16)
+constexpr int PFLUSH = 16;  // slabs between FP64 accumulations (k = 512)
+constexpr int PTHREADS = 512;
+
+__device__ inline void split3(float x, __bf16 &a, __bf16 &b, __bf16 &c) {
+    a = (__bf16) x;
+    const float r1 = x - (float) a;
+    b = (__bf16) r1;
+    c = (__bf16) (r1 - (float) b);
+}
+
+// tri: 1 when Y(k, j) = 0 for k > j, 2 when X(k, i) = 0 for k > i (k range of a tile
+// ends at its last column or row).  With tri == 2 on a 2-D grid the tile row is rotated
+// by the other grid coordinates, so that every XCD gets every row (the work of a tile
+// grows with its row).  lin: blockIdx.x runs over the lower tiles only.
+// OUT: 0 = FP32 C, 1 = FP64 C, 2 = three bf16 planes of C (at C + t * psC, ld ldc).
+template <int OUT, bool LOWER>
+__global__ __launch_bounds__(PTHREADS) void gemm_bf16p_kernel(int M, int N, int K, double alpha,
+                                                              const __bf16 *X, int ldx, long psX, long sX,
+                                                              const __bf16 *Y, int ldy, long psY, long sY,
+                                                              double beta, void *Cv, int ldc, long psC, long sC,
+                                                              int lin, int tri) {
+    int bx = blockIdx.x, by = blockIdx.y;
+    if (!lin && tri == 2) bx = (int) ((blockIdx.x + blockIdx.y + blockIdx.z) % gridDim.x);
+    if (lin) {
+        const int t = blockIdx.x;
+        int x = (int) ((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+        while ((x + 1) * (x + 2) / 2 <= t) ++x;
+        while (x * (x + 1) / 2 > t) --x;
+        bx = x;
+        by = t - x * (x + 1) / 2;
+    }
+    const int i0 = bx * PT, j0 = by * PT;
+    if (LOWER && i0 + PT - 1 < j0) return;
+    const int Ke = (tri == 1) ? min(K, j0 + PT) : (tri == 2) ? min(K, i0 + PT) : K;
+    X += (long) blockIdx.z * sX;
+    Y += (long) blockIdx.z * sY;
+    // planes [buffer][term][row][k]
+    __shared__ __attribute__((aligned(16))) __bf16 As[2][3][PT][PK + PPAD];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[2][3][PT][PK + PPAD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wi = (wave & 3) * 32, wj = (wave >> 2) * 64;
+    const int lr = lane & 15, lg = lane >> 4;
+    // loader: thread -> row tid / 4 of the tile, k group (tid & 3) * 8, one 16-byte
+    // vector per plane and operand
+    const int lrow = tid >> 2, lk = (tid & 3) * 8;
+    const bool xrow = i0 + lrow < M, yrow = j0 + lrow < N;
+    const __bf16 *xp = X + (long) (i0 + lrow) * ldx + lk;
+    const __bf16 *yp = Y + (long) (j0 + lrow) * ldy + lk;
+
+    f32x4 acc[2][4], cor[2][4];
+    double acc64[2][4][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+            cor[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc64[a][b][r] = 0.0;
+        }
+
+    u32x4 vx[3], vy[3];
+    auto fetch = [&](int k0) {
+        // k0 + lk .. k0 + lk + 7 lies inside [0, ld) whenever k0 + lk < Ke (Ke <= K, the
+        // padded length, a multiple of 8)
+        const bool kin = k0 + lk < Ke;
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            vx[t] = (xrow && kin) ? *reinterpret_cast<const u32x4 *>(xp + t * psX + k0) : (u32x4){0u, 0u, 0u, 0u};
+            vy[t] = (yrow && kin) ? *reinterpret_cast<const u32x4 *>(yp + t * psY + k0) : (u32x4){0u, 0u, 0u, 0u};
+        }
+    };
+    auto stash = [&](int buf) {
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            *reinterpret_cast<u32x4 *>(&As[buf][t][lrow][lk]) = vx[t];
+            *reinterpret_cast<u32x4 *>(&Bs[buf][t][lrow][lk]) = vy[t];
+        }
+    };
+
+    fetch(0);
+    stash(0);
+    __syncthreads();
+    const int nslab = (Ke + PK - 1) / PK;
+    for (int sl = 0; sl < nslab; ++sl) {
+        const int cur = sl & 1;
+        const bool more = sl + 1 < nslab;
+        if (more) fetch((sl + 1) * PK);
+        bf16x8 bfr[3][4];
+#pragma unroll
+        for (int t = 0; t < 3; ++t)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                bfr[t][b] = *reinterpret_cast<const bf16x8 *>(&Bs[cur][t][wj + 16 * b + lr][8 * lg]);
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            bf16x8 afr[3];
+#pragma unroll
+            for (int t = 0; t < 3; ++t)
+                afr[t] = *reinterpret_cast<const bf16x8 *>(&As[cur][t][wi + 16 * a + lr][8 * lg]);
+            // term-major over the four column blocks, so consecutive MFMAs accumulate into
+            // different registers; correction terms smallest first
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[0], bfr[0][b], acc[a][b], 0, 0, 0);
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                cor[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[2], bfr[0][b], cor[a][b], 0, 0, 0);
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                cor[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[0], bfr[2][b], cor[a][b], 0, 0, 0);
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                cor[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[1], bfr[1][b], cor[a][b], 0, 0, 0);
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                cor[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[1], bfr[0][b], cor[a][b], 0, 0, 0);
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                cor[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[0], bfr[1][b], cor[a][b], 0, 0, 0);
+        }
+        if ((sl + 1) % PFLUSH == 0 || !more) {
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) acc64[a][b][r] += (double) acc[a][b][r] + (double) cor[a][b][r];
+                    acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+                    cor[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+                }
+        }
+        if (more) stash(cur ^ 1);
+        __syncthreads();
+    }
+    // epilogue: lane holds rows i0 + wi + 16 a + 4 lg + r of column j0 + wj + 16 b + lr
+    if constexpr (OUT == 2) {
+        __bf16 *C = static_cast<__bf16 *>(Cv) + (long) blockIdx.z * sC;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = i0 + wi + 16 * a + 4 * lg + r, j = j0 + wj + 16 * b + lr;
+                    if (i < M && j < N && (!LOWER || i >= j)) {
+                        __bf16 p0, p1, p2;
+                        split3((float) (alpha * acc64[a][b][r]), p0, p1, p2);
+                        const long e = i + (long) j * ldc;
+                        C[e] = p0;
+                        C[e + psC] = p1;
+                        C[e + 2 * psC] = p2;
+                    }
+                }
+    } else {
+        typedef typename std::conditional<OUT == 1, double, float>::type Tout;
+        Tout *C = static_cast<Tout *>(Cv) + (long) blockIdx.z * sC;
+        Tout cin[2][4][4];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = i0 + wi + 16 * a + 4 * lg + r, j = j0 + wj + 16 * b + lr;
+                    const bool ok = i < M && j < N && (!LOWER || i >= j);
+                    cin[a][b][r] = (ok && beta != 0.0) ? C[i + (long) j * ldc] : (Tout) 0;
+                }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = i0 + wi + 16 * a + 4 * lg + r, j = j0 + wj + 16 * b + lr;
+                    if (i < M && j < N && (!LOWER || i >= j))
+                        C[i + (long) j * ldc] = (Tout) (alpha * acc64[a][b][r] + beta * (double) cin[a][b][r]);
+                }
+    }
+}
+
+template <int OUT>
+void launch_p(hipStream_t s, bool lower, int M, int N, int K, double alpha, const __bf16 *X, int ldx, long psX,
+              long sX, const __bf16 *Y, int ldy, long psY, long sY, double beta, void *C, int ldc, long psC, long sC,
+              int batch, int tri) {
+    const int mt = (M + PT - 1) / PT, nt = (N + PT - 1) / PT;
+    const int lin = (lower && mt == nt) ? 1 : 0;
+    dim3 grid(lin ? mt * (mt + 1) / 2 : mt, lin ? 1 : nt, batch);
+    if (lower)
+        gemm_bf16p_kernel<OUT, true><<<grid, PTHREADS, 0, s>>>(M, N, K, alpha, X, ldx, psX, sX, Y, ldy, psY, sY, beta,
+                                                               C, ldc, psC, sC, lin, tri);
+    else
+        gemm_bf16p_kernel<OUT, false><<<grid, PTHREADS, 0, s>>>(M, N, K, alpha, X, ldx, psX, sX, Y, ldy, psY, sY,
+                                                                beta, C, ldc, psC, sC, 0, tri);
+}
+
+// Three bf16 planes of a rows x cols FP64 (or FP32) matrix, rows rows..ld-1 of the planes
+// zero (the k padding of a product operand).
+template <typename T>
+__global__ void split_matrix_kernel(int rows, int cols, const T *A, long lda, __bf16 *P, int ld, long ps) {
+    const long total = (long) ld * cols;
+    for (long e = blockIdx.x * (long) blockDim.x + threadIdx.x; e < total; e += (long) gridDim.x * blockDim.x) {
+        const int i = (int) (e % ld), j = (int) (e / ld);
+        __bf16 p0 = (__bf16) 0.0f, p1 = p0, p2 = p0;
+        if (i < rows) split3((float) A[i + (long) j * lda], p0, p1, p2);
+        P[e] = p0;
+        P[e + ps] = p1;
+        P[e + 2 * ps] = p2;
+    }
+}
+
+// Full n x n planes (ld ldn) of the packed dense coefficients slot0 .. slot0 + cnt - 1,
+// one workgroup per 64 x 64 lower tile, the mirror written through LDS (both writes
+// coalesced along the rows).
+__global__ __launch_bounds__(256) void unpack_dense_split_kernel(int n, const double *dVal, int slot0, __bf16 *out,
+                                                                 int ldn, long ps, long stride) {
+    __shared__ float tile[64][65];  // tile[c][r] = A(i0 + r, j0 + c)
+    const long npk = (long) n * (n + 1) / 2;
+    const double *v = dVal + (long) (slot0 + blockIdx.y) * npk;
+    __bf16 *o = out + (long) blockIdx.y * stride;
+    const int t = blockIdx.x;
+    int bi = (int) ((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
+    while (bi * (bi + 1) / 2 > t) --bi;
+    const int bj = t - bi * (bi + 1) / 2;
+    const int i0 = bi * 64, j0 = bj * 64, tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int c = ty; c < 64; c += 4) {
+        const long i = i0 + tx, j = j0 + c;
+        if (i < n && j < n && i >= j) {
+            const float val = (float) v[j * n - j * (j + 1) / 2 + i];
+            __bf16 p0, p1, p2;
+            split3(val, p0, p1, p2);
+            const long e = i + j * ldn;
+            o[e] = p0;
+            o[e + ps] = p1;
+            o[e + 2 * ps] = p2;
+            tile[c][tx] = val;
+        }
+    }
+    __syncthreads();
+    for (int c = ty; c < 64; c += 4) {
+        // mirror entry (j0 + tx, i0 + c) = A(i0 + c, j0 + tx) = tile[tx][c], above the diagonal only
+        const long r = j0 + tx, q = i0 + c;
+        if (r < n && q < n && q > r) {
+            __bf16 p0, p1, p2;
+            split3(tile[tx][c], p0, p1, p2);
+            const long e = r + q * ldn;
+            o[e] = p0;
+            o[e + ps] = p1;
+            o[e + 2 * ps] = p2;
+        }
+    }
+}
+
+// Strictly lower part of each symmetric T_p (n x n FP32, lower valid) packed column by
+// column into planes of off (column p at p * ldo, entry j n - j (j + 1) / 2 + (i - j - 1)
+// for i > j) and its diagonal into planes of dg (column p at p * ldd).
+__global__ void pack_lower_split_kernel(int n, const float *Tin, long strideIn, __bf16 *off, long ldo, long pso,
+                                        __bf16 *dg, int ldd, long psd) {
+    const int p = blockIdx.y;
+    const long nn = (long) n * n;
+    const float *src = Tin + p * strideIn;
+    for (long e = blockIdx.x * (long) blockDim.x + threadIdx.x; e < nn; e += (long) gridDim.x * blockDim.x) {
+        const long i = e % n, j = e / n;
+        if (i < j) continue;
+        __bf16 p0, p1, p2;
+        split3(src[e], p0, p1, p2);
+        __bf16 *o;
+        long ps;
+        if (i > j) {
+            o = off + p * ldo + j * n - j * (j + 1) / 2 + (i - j - 1);
+            ps = pso;
+        } else {
+            o = dg + (long) p * ldd + i;
+            ps = psd;
+        }
+        o[0] = p0;
+        o[ps] = p1;
+        o[2 * ps] = p2;
+    }
+}
+
+}  // namespace
+
+void gemm_bf16p(hipStream_t s, bool lower, int M, int N, int K, double alpha, const __bf16 *X, int ldx, long psX,
+                long sX, const __bf16 *Y, int ldy, long psY, long sY, double beta, void *C, int outKind, int ldc,
+                long psC, long sC, int batch, int tri) {
+    if (M <= 0 || N <= 0 || batch <= 0) return;
+    // every operand load is one aligned 16-byte vector inside [0, K) of a column
+    const bool aligned = K % 8 == 0 && ldx % 8 == 0 && ldy % 8 == 0 && psX % 8 == 0 && psY % 8 == 0 &&
+                         sX % 8 == 0 && sY % 8 == 0 && ((uintptr_t) X % 16) == 0 && ((uintptr_t) Y % 16) == 0 &&
+                         K <= ldx && K <= ldy;
+    if (!aligned) {
+        fprintf(stderr, "[hdsdp_amd] gemm_bf16p: operand layout not 8-element aligned (K %d ldx %d ldy %d); "
+                        "product skipped\n", K, ldx, ldy);
+        return;
+    }
+    if (outKind == BF16P_OUT_F64)
+        launch_p<1>(s, lower, M, N, K, alpha, X, ldx, psX, sX, Y, ldy, psY, sY, beta, C, ldc, psC, sC, batch, tri);
+    else if (outKind == BF16P_OUT_F32)
+        launch_p<0>(s, lower, M, N, K, alpha, X, ldx, psX, sX, Y, ldy, psY, sY, beta, C, ldc, psC, sC, batch, tri);
+    else
+        launch_p<2>(s, lower, M, N, K, alpha, X, ldx, psX, sX, Y, ldy, psY, sY, beta, C, ldc, psC, sC, batch, tri);
+}
+
+void split_matrix(hipStream_t s, int rows, int cols, const double *A, long lda, __bf16 *P, int ld, long ps) {
+    const long total = (long) ld * cols;
+    const int blocks = (int) std::min<long>(4096, (total + 255) / 256);
+    if (blocks > 0) split_matrix_kernel<double><<<blocks, 256, 0, s>>>(rows, cols, A, lda, P, ld, ps);
+}
+
+void split_matrix_f(hipStream_t s, int rows, int cols, const float *A, long lda, __bf16 *P, int ld, long ps) {
+    const long total = (long) ld * cols;
+    const int blocks = (int) std::min<long>(4096, (total + 255) / 256);
+    if (blocks > 0) split_matrix_kernel<float><<<blocks, 256, 0, s>>>(rows, cols, A, lda, P, ld, ps);
+}
+
+void unpack_dense_split(hipStream_t s, const DevCoeffs &c, int slot0, int cnt, __bf16 *out, int ldn, long ps,
+                        long stride) {
+    if (cnt <= 0) return;
+    const int nt = (c.n + 63) / 64;
+    unpack_dense_split_kernel<<<dim3(nt * (nt + 1) / 2, cnt), 256, 0, s>>>(c.n, c.dVal, slot0, out, ldn, ps, stride);
+}
+
+void pack_lower_split(hipStream_t s, int n, int cnt, const float *Tin, long strideIn, __bf16 *off, long ldo, long pso,
+                      __bf16 *dg, int ldd, long psd) {
+    if (cnt <= 0) return;
+    pack_lower_split_kernel<<<dim3(1024, cnt), 256, 0, s>>>(n, Tin, strideIn, off, ldo, pso, dg, ldd, psd);
+}
+
+}  // namespace hdk

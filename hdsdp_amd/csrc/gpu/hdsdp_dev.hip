@@ -9,6 +9,7 @@
 #include "dev_common.h"
 #include "schur.h"
 #include "vec.h"
+#include "bf16planes.h"
 #include "eig_small.h"
 
 #include <algorithm>
@@ -46,6 +47,12 @@ struct hdsdp_dev {
     float *LiTf;             // n x n, L^-T of the dual factor
     float *Ff, *Gf;          // n x nR1: rank-one factors and G = L^-1 F
     float *Tf, *Tdgf, *Dbuff, *Vf;  // mixed: packed T_p and diagonals, batch buffers
+    // mixed dense Gram from bf16 planes (three per operand, split once by the producing
+    // kernel): L^-T, a batch of A_p and of V_p = A_p L^-T (ld plane_ld(n)), the packed T_p
+    // (ld K1p, a multiple of 32, entries K1..K1p zero) and their diagonals
+    __bf16 *LiTp, *Dp, *Vp, *Tp, *Tdgp;
+    long K1p, chunkP;
+    int planes;
     // adaptive precision: the Gram blocks of M come from bf16 split products while
     // schurMixed; when the system they form cannot be factored or solved, they are
     // rebuilt in FP64 from the S^-1 and L^-1 of their build (Wm, Lim), the diagonal
@@ -75,7 +82,9 @@ struct hdsdp_dev {
     std::vector<double> Hh;
     std::vector<double> lzStart, lzPert;  // Lanczos start vector and warm-start perturbation (fixed by n)
 
-    // Schur system
+    // Schur system; mVer counts changes of M, cholVer is the mVer its FP64 factor
+    // (Mchol, MLi, MLiT) was computed from (-1: none)
+    long mVer, cholVer;
     double *M, *Mchol, *MLi, *MLiT;
     float *Mf, *MLif, *MLiTf, *cf1, *cf2, *workf;
     double *jac, *cr, *crn, *cp, *cz, *cMp, *cx, *crhs, *ctmp;
@@ -84,7 +93,7 @@ struct hdsdp_dev {
     int precKind;
     int useJacobi, paramUseJacobi, maxIter, nRestart;
     double absTol, relTol;
-    double stats[6];
+    double stats[7];
 
     // device-time profile (events on st)
     bool prof;
@@ -110,6 +119,15 @@ hdsdp_retcode dalloc(hdsdp_dev *d, T **p, size_t count) {
     }
     d->allocs.push_back(q);
     *p = (T *) q;
+    return HDSDP_RETCODE_OK;
+}
+
+// device allocation cleared to zero on the library's stream
+template <typename T>
+hdsdp_retcode dalloc_zero(hdsdp_dev *d, T **p, size_t count) {
+    hdsdp_retcode rc = dalloc(d, p, count);
+    if (rc != HDSDP_RETCODE_OK) return rc;
+    HIP_OK(hipMemsetAsync(*p, 0, sizeof(T) * std::max<size_t>(count, 1), d->st));
     return HDSDP_RETCODE_OK;
 }
 
@@ -310,6 +328,7 @@ hdsdp_retcode build_chol64(hdsdp_dev *d) {
         DCALL(dalloc(d, &d->MLiT, std::max((size_t) m * m, hdk::POTRF_WORK)));
     }
     HIP_OK(hipMemcpyAsync(d->Mchol, d->M, sizeof(double) * (size_t) m * m, hipMemcpyDeviceToDevice, d->st));
+    d->cholVer = -1;
     if (!hdk::potrf<double>(d->st, m, d->Mchol, m, d->MLiT, std::max((size_t) m * m, hdk::POTRF_WORK), d->info_d,
                             &d->la))
         return HDSDP_RETCODE_FAILED;
@@ -319,6 +338,7 @@ hdsdp_retcode build_chol64(hdsdp_dev *d) {
     hdk::trtri<double>(d->st, m, d->Mchol, m, d->MLi, m, d->MLiT);
     hdk::transpose<double>(d->st, m, d->MLi, m, d->MLiT, m);
     d->precKind = PREC_CHOL64;
+    d->cholVer = d->mVer;
     return check_launch();
 }
 
@@ -567,6 +587,7 @@ void dense_gram(hdsdp_dev *d, const double *Li) {
 // diagonal terms added since, and keep later builds in FP64.
 hdsdp_retcode rebuild_gram_fp64(hdsdp_dev *d) {
     const int m = d->m, n = d->n;
+    d->mVer += 1;
     d->schurMixed = 0;
     d->mGramPending = 0;
     hdk::get_diag(d->st, m, d->M, m, d->mDiagCur);
@@ -600,7 +621,7 @@ hdsdp_retcode rebuild_gram_fp64(hdsdp_dev *d) {
 // Mixed precision: the same products on the bf16 matrix cores (FP32-equivalent split
 // products), written as X^T Y with both operands contiguous along k:
 //   V_p = A_p L^-T = (A_p)^T (L^-T),  T_p = L^-1 V_p = (L^-T)^T V_p,  M = T^T T.
-void dense_gram_mixed(hdsdp_dev *d) {
+void dense_gram_mixed_inloop(hdsdp_dev *d) {
     const int n = d->n, r = d->nDn, m = d->m;
     const long nn = (long) n * n, K1 = (long) n * (n - 1) / 2;
     hdk::d2f(d->st, nn, d->LiT[0], d->LiTf);
@@ -629,6 +650,50 @@ void dense_gram_mixed(hdsdp_dev *d) {
                                    d->Tf + (long) full * kc, (int) K1, 0, 0.0, d->Gpart + (long) full * rr, r, 0, 1);
     hdk::gemm_bf16s_tn<double>(d->st, BF16_TERMS, true, r, r, n, 1.0, d->Tdgf, n, 0, d->Tdgf, n, 0, 0.0,
                                d->Gpart + (long) nOff * rr, r, 0, 1);
+    hdk::gram_reduce(d->st, r, nOff + 1, d->Gpart, r, rr, d->c.denseCon, d->M, m);
+}
+
+// Mixed precision: the same products on the bf16 matrix cores (FP32-equivalent split
+// products), written as X^T Y with both operands contiguous along k:
+//   V_p = A_p L^-T = (A_p)^T (L^-T),  T_p = L^-1 V_p = (L^-T)^T V_p,  M = T^T T.
+// Every operand is held as three bf16 planes written by the kernel that produces it
+// (L^-T and A_p by split kernels, V_p by the first product's epilogue, T_p by the pack).
+void dense_gram_mixed(hdsdp_dev *d) {
+    const int n = d->n, r = d->nDn, m = d->m;
+    const long nn = (long) n * n;
+    if (!d->planes) {
+        dense_gram_mixed_inloop(d);
+        return;
+    }
+    const int ldn = (int) hdk::plane_ld(n);
+    const long nnp = (long) ldn * n, bps = nnp * d->gramBatch, K1p = d->K1p, rr = (long) r * r;
+    hdk::split_matrix(d->st, n, n, d->LiT[0], n, d->LiTp, ldn, nnp);
+    for (int b0 = 0; b0 < r; b0 += d->gramBatch) {
+        const int cnt = std::min(d->gramBatch, r - b0);
+        hdk::unpack_dense_split(d->st, d->c, b0, cnt, d->Dp, ldn, bps, nnp);
+        // V = A_p L^-T: column j of L^-T is zero below row j (tri 1); written as planes
+        hdk::gemm_bf16p(d->st, false, n, n, ldn, 1.0, d->Dp, ldn, bps, nnp, d->LiTp, ldn, nnp, 0, 0.0, d->Vp,
+                        hdk::BF16P_OUT_PLANES, ldn, bps, nnp, cnt, 1);
+        // lower triangle of T_p = (L^-T)^T V only (symmetric); column i of the left
+        // operand L^-T is zero below row i (tri 2)
+        hdk::gemm_bf16p(d->st, true, n, n, ldn, 1.0, d->LiTp, ldn, nnp, 0, d->Vp, ldn, bps, nnp, 0.0, d->Dbuff,
+                        hdk::BF16P_OUT_F32, n, 0, nn, cnt, 2);
+        hdk::pack_lower_split(d->st, n, cnt, d->Dbuff, nn, d->Tp + (size_t) b0 * K1p, K1p, K1p * r,
+                              d->Tdgp + (size_t) b0 * ldn, ldn, (long) ldn * r);
+    }
+    const long kc = d->chunkP;
+    const int full = (int) (K1p / kc);
+    const long rem = K1p - (long) full * kc;
+    const int nOff = full + (rem > 0 ? 1 : 0);
+    if (full > 0)
+        hdk::gemm_bf16p(d->st, true, r, r, (int) kc, 2.0, d->Tp, (int) K1p, K1p * r, kc, d->Tp, (int) K1p, K1p * r, kc,
+                        0.0, d->Gpart, hdk::BF16P_OUT_F64, r, 0, rr, full, 0);
+    if (rem > 0)
+        hdk::gemm_bf16p(d->st, true, r, r, (int) rem, 2.0, d->Tp + (long) full * kc, (int) K1p, K1p * r, 0,
+                        d->Tp + (long) full * kc, (int) K1p, K1p * r, 0, 0.0, d->Gpart + (long) full * rr,
+                        hdk::BF16P_OUT_F64, r, 0, 0, 1, 0);
+    hdk::gemm_bf16p(d->st, true, r, r, ldn, 1.0, d->Tdgp, ldn, (long) ldn * r, 0, d->Tdgp, ldn, (long) ldn * r, 0, 0.0,
+                    d->Gpart + (long) nOff * rr, hdk::BF16P_OUT_F64, r, 0, 0, 1, 0);
     hdk::gram_reduce(d->st, r, nOff + 1, d->Gpart, r, rr, d->c.denseCon, d->M, m);
 }
 
@@ -724,6 +789,8 @@ hdsdp_retcode hdsdp_dev_create(hdsdp_dev **pdev, int nRow, int nCol, int precisi
     d->absTol = 1e-06;
     d->relTol = 1e-06;
     d->useJacobi = d->paramUseJacobi = 1;
+    d->mVer = 0;
+    d->cholVer = -1;
     d->maxIter = std::max(50, nRow / 20);
     d->nRestart = 20;
     hdk::fill(d->st, d->M, (long) m * m, 0.0);
@@ -938,6 +1005,8 @@ hdsdp_retcode hdsdp_dev_load(hdsdp_dev *d, const hdsdp_coeff_view *cv) {
     if (c.nR1 > 0 || c.cDSR1) DCALL(dalloc(d, &d->H1, (size_t) n * (c.nR1 + 1)));
     if (d->r1Gram && !d->r1Identity) DCALL(dalloc(d, &d->Kr, (size_t) c.nR1 * c.nR1));
     d->LiTf = d->Ff = d->Gf = d->Tf = d->Tdgf = d->Dbuff = d->Vf = nullptr;
+    d->LiTp = d->Dp = d->Vp = d->Tp = d->Tdgp = nullptr;
+    d->K1p = d->chunkP = 0;
     d->Wm = d->Lim = d->mDiag0 = d->mDiagCur = d->xCoef = nullptr;
     // bf16-split Gram blocks need the matrix-free FP64 product in the refinement.  For dense
     // constraints it costs two passes over the packed coefficients and two n^3 products,
@@ -950,8 +1019,12 @@ hdsdp_retcode hdsdp_dev_load(hdsdp_dev *d, const hdsdp_coeff_view *cv) {
         mixedGram = d->precision == HDSDP_PRECISION_MIXED && (d->r1Gram || d->dnGram) && atoi(force) != 0;
     d->schurMixed = mixedGram ? 1 : 0;
     d->mGramPending = 0;
+    // bf16 planes for the dense mixed Gram (split once by the producing kernels) or the
+    // split-in-the-loop products; HDSDP_BF16_PLANES=1 selects the planes
+    d->planes = 0;
+    if (const char *pl = getenv("HDSDP_BF16_PLANES")) d->planes = atoi(pl) != 0;
     if (mixedGram) {
-        DCALL(dalloc(d, &d->LiTf, (size_t) n * n));
+        if (d->r1Gram || !d->planes) DCALL(dalloc(d, &d->LiTf, (size_t) n * n));
         DCALL(dalloc(d, &d->Wm, (size_t) n * n));
         DCALL(dalloc(d, &d->Lim, (size_t) n * n));
         DCALL(dalloc(d, &d->mDiag0, (size_t) m));
@@ -981,7 +1054,21 @@ hdsdp_retcode hdsdp_dev_load(hdsdp_dev *d, const hdsdp_coeff_view *cv) {
         d->gramChunk = std::max<long>(1, (K % parts == 0) ? K / parts : std::max<long>(256, (K + parts - 1) / parts));
         d->gramParts = (int) ((K + d->gramChunk - 1) / d->gramChunk);
         const size_t K1 = std::max<size_t>(1, (size_t) n * (n - 1) / 2);
-        if (mixedGram) {
+        if (mixedGram && d->planes) {
+            // planes: operands zero outside their true extent (k padding of the products)
+            const size_t ldn = (size_t) hdk::plane_ld(n), nnp = ldn * n;
+            d->K1p = (long) (K1 + 31) / 32 * 32;
+            const long tilesP = tiles, wantP = std::min<long>(16, std::max<long>(1, (2048 + tilesP - 1) / tilesP));
+            d->chunkP = std::max<long>(32, (d->K1p / wantP + 31) / 32 * 32);
+            // Gpart holds the parts of either build (the FP64 rebuild uses gramChunk)
+            d->gramParts = std::max<int>(d->gramParts, (int) ((d->K1p + d->chunkP - 1) / d->chunkP));
+            DCALL(dalloc_zero(d, &d->LiTp, 3 * nnp));
+            DCALL(dalloc_zero(d, &d->Dp, 3 * nnp * d->gramBatch));
+            DCALL(dalloc_zero(d, &d->Vp, 3 * nnp * d->gramBatch));
+            DCALL(dalloc_zero(d, &d->Tp, 3 * (size_t) d->K1p * r));
+            DCALL(dalloc_zero(d, &d->Tdgp, 3 * ldn * r));
+            DCALL(dalloc(d, &d->Dbuff, nn * d->gramBatch));
+        } else if (mixedGram) {
             DCALL(dalloc(d, &d->Tf, K1 * r));
             DCALL(dalloc(d, &d->Tdgf, (size_t) n * r));
             DCALL(dalloc(d, &d->Dbuff, nn * d->gramBatch));
@@ -1071,7 +1158,7 @@ void hdsdp_dev_destroy(hdsdp_dev **pdev) {
 }
 
 hdsdp_retcode hdsdp_dev_stats(hdsdp_dev *d, double *stats, int nStats) {
-    for (int k = 0; k < nStats && k < 6; ++k) stats[k] = d->stats[k];
+    for (int k = 0; k < nStats && k < 7; ++k) stats[k] = d->stats[k];
     return HDSDP_RETCODE_OK;
 }
 
@@ -1184,11 +1271,13 @@ hdsdp_retcode hdsdp_dev_schur(hdsdp_dev *d, int typeKKT, double Rd, double *asin
         DCALL(prof_end(d, HDSDP_PROF_PAIRS));
         const bool mixed = d->schurMixed != 0;
         d->mGramPending = mixed ? 1 : 0;
+        d->mVer += 1;
         if (mixed) {
             // keep the factor this M is built from, for an FP64 rebuild
             HIP_OK(hipMemcpyAsync(d->Wm, d->W, sizeof(double) * (size_t) n * n, hipMemcpyDeviceToDevice, d->st));
             HIP_OK(hipMemcpyAsync(d->Lim, d->Li[0], sizeof(double) * (size_t) n * n, hipMemcpyDeviceToDevice, d->st));
         }
+        if (d->r1Gram || d->dnGram) prof_begin(d, HDSDP_PROF_GRAM);
         if (d->r1Gram) {
             if (mixed) r1_gram_mixed(d);
             else if (d->r1Identity) hdk::r1_schur(d->st, d->c, d->H1, d->M, m, d->M, m);
@@ -1198,6 +1287,7 @@ hdsdp_retcode hdsdp_dev_schur(hdsdp_dev *d, int typeKKT, double Rd, double *asin
             if (mixed) dense_gram_mixed(d);
             else dense_gram(d, d->Li[0]);
         }
+        if (d->r1Gram || d->dnGram) DCALL(prof_end(d, HDSDP_PROF_GRAM));
         for (int h : d->heavy) {
             // B_h = W A_h W, then M(h, k) = <A_k, B_h>
             hdk::densify(d->st, d->c, h, d->T1);
@@ -1235,12 +1325,14 @@ hdsdp_retcode hdsdp_dev_kkt_params(hdsdp_dev *d, double relTol, double absTol, i
 }
 
 hdsdp_retcode hdsdp_dev_kkt_add_diag(hdsdp_dev *d, const double *diag) {
+    d->mVer += 1;
     DCALL(upload(d, d->ctmp, diag, d->m, d->hvecA));
     hdk::add_diag(d->st, d->m, d->M, d->m, d->ctmp, 0.0);
     return check_launch();
 }
 
 hdsdp_retcode hdsdp_dev_kkt_regularize(hdsdp_dev *d, double reg) {
+    d->mVer += 1;
     hdk::regularize(d->st, d->m, d->M, d->m, reg);
     return check_launch();
 }
@@ -1269,26 +1361,73 @@ hdsdp_retcode hdsdp_dev_kkt_solve(hdsdp_dev *d, const double *rhs, double *sol) 
     return download(d, sol, d->cx, d->m, d->hvecB);
 }
 
+// z = (L L^T)^-1 r with the FP64 factor of M (MLi = L^-1, MLiT = L^-T); tmp: m-vector.
+void apply_chol64(hdsdp_dev *d, const double *r, double *z, double *tmp) {
+    const int m = d->m;
+    hdk::coldot<double>(d->st, m, m, 2, 1.0, d->MLiT, m, r, 0.0, tmp);
+    hdk::coldot<double>(d->st, m, m, 1, 1.0, d->MLi, m, tmp, 0.0, z);
+}
+
 // Direct FP64 solve of M x = rhs (FP64 Gram blocks, FP64 Cholesky, two refinement steps
 // on the residual with M), for the centring and adjoint solves, which need the solution
 // to working accuracy rather than to the interior-point tolerance.
-hdsdp_retcode hdsdp_dev_kkt_solve_exact(hdsdp_dev *d, const double *rhs, double *sol) {
+//
+// reuse: when the FP64 factor on the device belongs to an earlier M (the last centring
+// Newton step factored the Schur matrix of the point before the centred one), refine
+// against the current M with that factor as the preconditioner instead of factoring
+// again: x = P^-1 b, then x += P^-1 (b - M x) until the correction stops shrinking (at
+// most eight steps).  Kept only if the factor contracts the first correction tenfold;
+// otherwise (a factor of a different matrix) M is factored afresh.
+hdsdp_retcode solve_exact(hdsdp_dev *d, const double *rhs, double *sol, bool reuse) {
     const int m = d->m;
     if (d->mGramPending) DCALL(rebuild_gram_fp64(d));
     hdk::mirror_lower<double>(d->st, m, d->M, m);
-    prof_begin(d, HDSDP_PROF_MFACTOR);
-    DCALL(build_chol64(d));
-    DCALL(prof_end(d, HDSDP_PROF_MFACTOR));
     DCALL(upload(d, d->crhs, rhs, m, d->hvecB));
-    apply_precond(d, d->crhs, d->cx, nullptr);
+    if (reuse && d->cholVer >= 0 && d->cholVer != d->mVer) {
+        apply_chol64(d, d->crhs, d->cx, d->ctmp);
+        double z0 = -1.0, zprev = HUGE_VAL;
+        bool ok = false;
+        for (int it = 0; it < 8; ++it) {
+            symv_M(d, d->cx, d->crn);
+            hdk::cg_residual(d->st, m, d->crhs, d->crn, d->cr);
+            apply_chol64(d, d->cr, d->cz, d->ctmp);
+            hdk::axpy_norm(d->st, m, 0.0, d->cx, d->cz, d->sc_d, hdk::SC_AUX);   // ||z||
+            hdk::axpy_norm(d->st, m, 1.0, d->cz, d->cx, d->sc_d, hdk::CG_TRUE);  // x += z, ||x||
+            DCALL(read_scalars(d));
+            const double zn = d->hsc[hdk::SC_AUX], xn = d->hsc[hdk::CG_TRUE];
+            if (!(zn == zn) || !(xn == xn)) break;
+            if (it == 0) z0 = zn;
+            if (it == 1 && zn > 0.1 * z0) break;  // not a factor of a nearby matrix
+            if (zn <= 1e-15 * xn || (it >= 1 && zn > 0.5 * zprev)) {
+                ok = it >= 1;
+                break;
+            }
+            zprev = zn;
+        }
+        if (ok) {
+            d->stats[6] += 1;
+            DCALL(check_launch());
+            return download(d, sol, d->cx, m, d->hvecB);
+        }
+    }
+    if (d->cholVer != d->mVer) {
+        prof_begin(d, HDSDP_PROF_MFACTOR);
+        DCALL(build_chol64(d));
+        DCALL(prof_end(d, HDSDP_PROF_MFACTOR));
+    }
+    apply_chol64(d, d->crhs, d->cx, d->ctmp);
     for (int it = 0; it < 2; ++it) {
-        symv_M(d, d->cx, d->ctmp);
-        hdk::cg_residual(d->st, m, d->crhs, d->ctmp, d->cr);
-        apply_precond(d, d->cr, d->cz, nullptr);
+        symv_M(d, d->cx, d->crn);
+        hdk::cg_residual(d->st, m, d->crhs, d->crn, d->cr);
+        apply_chol64(d, d->cr, d->cz, d->ctmp);
         hdk::axpy(d->st, m, 1.0, d->cz, d->cx);
     }
     DCALL(check_launch());
     return download(d, sol, d->cx, m, d->hvecB);
+}
+
+hdsdp_retcode hdsdp_dev_kkt_solve_exact(hdsdp_dev *d, const double *rhs, double *sol) {
+    return solve_exact(d, rhs, sol, false);
 }
 
 hdsdp_retcode hdsdp_dev_get_schur(hdsdp_dev *d, double *M) {
@@ -1356,7 +1495,7 @@ hdsdp_retcode hdsdp_dev_get_dual(hdsdp_dev *d, double *S) {
 hdsdp_retcode hdsdp_dev_adjoint(hdsdp_dev *d, double mu, const double *dLdy, double *dLdb, double *dLdC) {
     const int m = d->m, n = d->n;
     std::vector<double> lam(m, 0.0);
-    DCALL(hdsdp_dev_kkt_solve_exact(d, dLdy, lam.data()));
+    DCALL(solve_exact(d, dLdy, lam.data(), true));
     for (int i = 0; i < m; ++i) dLdb[i] = lam[i] / mu;
     // dLdC = W (sum_i lambda_i A_i) W with W = S^-1 of the dual factor
     DCALL(ensure_linv(d, 0));

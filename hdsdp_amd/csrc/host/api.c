@@ -7,6 +7,7 @@
  * with HDSDP_RETCODE_FAILED.
  */
 #include "hdsdp_host.h"
+#include "eig_small.h"
 
 /* ---- user data (interface/hdsdp_user_data.c:38-121) ---------------------- */
 
@@ -450,6 +451,53 @@ void HDSDPGetConeValues( hdsdp *s, int iCone, double *conePrimal, double *coneDu
     if ( coneDual ) hdsdp_dev_get_dual(s->sdp->dev, coneDual);
 }
 
+/* Largest eigenvalue of the symmetric n x n matrix X (full storage): Lanczos with full
+ * reorthogonalisation from the normalised ones vector, at most HDSDP_EIG_SMALL_MAX steps
+ * (exact up to rounding for n <= HDSDP_EIG_SMALL_MAX; beyond, the top Ritz value of that
+ * Krylov space), then the top eigenvalue of the tridiagonal matrix. */
+static double sym_lambda_max( int n, const double *X ) {
+    const int kmax = n < HDSDP_EIG_SMALL_MAX ? n : HDSDP_EIG_SMALL_MAX;
+    double *V = (double *) calloc((size_t) n * (kmax + 1), sizeof(double));
+    double *w = (double *) calloc((size_t) n, sizeof(double));
+    double alpha[HDSDP_EIG_SMALL_MAX], beta[HDSDP_EIG_SMALL_MAX], ev = 0.0;
+    int k = 0;
+    if ( !V || !w ) goto done;
+    for ( int i = 0; i < n; ++i ) V[i] = 1.0 / sqrt((double) n);
+    for ( k = 0; k < kmax; ++k ) {
+        const double *v = V + (size_t) k * n;
+        for ( int i = 0; i < n; ++i ) {
+            double acc = 0.0;
+            for ( int j = 0; j < n; ++j ) acc += X[i + (size_t) j * n] * v[j];
+            w[i] = acc;
+        }
+        double a = 0.0;
+        for ( int i = 0; i < n; ++i ) a += w[i] * v[i];
+        alpha[k] = a;
+        /* full reorthogonalisation against every Lanczos vector, twice */
+        for ( int pass = 0; pass < 2; ++pass )
+            for ( int q = 0; q <= k; ++q ) {
+                const double *u = V + (size_t) q * n;
+                double c = 0.0;
+                for ( int i = 0; i < n; ++i ) c += w[i] * u[i];
+                for ( int i = 0; i < n; ++i ) w[i] -= c * u[i];
+            }
+        double b = 0.0;
+        for ( int i = 0; i < n; ++i ) b += w[i] * w[i];
+        b = sqrt(b);
+        if ( k + 1 == kmax || b <= 1e-14 * fabs(a) + 1e-300 ) {
+            ++k;
+            break;
+        }
+        beta[k] = b;
+        for ( int i = 0; i < n; ++i ) V[(size_t) (k + 1) * n + i] = w[i] / b;
+    }
+    if ( hdsdp_eig_tridiag_top(k, alpha, beta, 1, &ev, NULL) != 0 ) ev = alpha[0];
+done:
+    free(V);
+    free(w);
+    return ev;
+}
+
 /* interface/hdsdp.c:753-906 */
 hdsdp_retcode HDSDPCheckSolution( hdsdp *s, double dErrs[6] ) {
     hdsdp_retcode retcode = HDSDP_RETCODE_OK;
@@ -471,9 +519,18 @@ hdsdp_retcode HDSDPCheckSolution( hdsdp *s, double dErrs[6] ) {
     HCALL(recover_primal(s, &ok));
     if ( ok ) {
         HCALL(hdsdp_dev_primal_stats(s->sdp->dev, AX, &primalObj, &compl, &traceX));
-        /* The reference takes the largest eigenvalue of X here (dsyevr with
-         * il = iu = n); a positive trace already makes it positive. */
-        maxEig = ( traceX > 0.0 ) ? traceX / s->sdp->nCol : traceX;
+        /* The reference takes the largest eigenvalue of X here (fds_syev with m = 1:
+         * dsyevr, il = iu = n, linalg/dense_opts.c:53-73): a positive trace makes it
+         * positive (lambda_max >= tr X / n) and the error zero; otherwise it is computed. */
+        if ( traceX > 0.0 ) {
+            maxEig = traceX / s->sdp->nCol;
+        } else {
+            const int n = s->sdp->nCol;
+            double *X = NULL;
+            HALLOC(X, double, (size_t) n * n);
+            if ( hdsdp_dev_get_primal(s->sdp->dev, X) == HDSDP_RETCODE_OK ) maxEig = sym_lambda_max(n, X);
+            free(X);
+        }
     } else {
         hlog(s, "Recovery step is infeasible\n");
     }

@@ -27,6 +27,8 @@ __all__ = [
     "rank_one",
     "write_instance",
     "read_instance",
+    "write_sdpa",
+    "read_sdpa",
     "with_data",
     "CONFIGS",
 ]
@@ -278,3 +280,53 @@ def read_instance(path: str, name: str = "file") -> SDPInstance:
         elem = np.frombuffer(f.read(8 * nnz), dtype="<f8").copy()
         b = np.frombuffer(f.read(8 * m), dtype="<f8").copy()
     return SDPInstance(name, n, m, beg, idx, elem, b)
+
+
+def write_sdpa(inst: SDPInstance, path: str, comment: str | None = None) -> None:
+    """SDPA sparse format of a one-block instance: ``F0 = -C``, ``F_i = A_i``, ``c = b``
+    (the SDPA primal ``min <F0, X>`` form the reference reads, negating F0 into C,
+    ``interface/hdsdp_file_io.c:234-236, 248-250``).  Entries are written upper-triangular
+    (row <= column), constraint by constraint in CSC order."""
+    if inst.factor is not None:
+        raise ValueError("factor-form instances have no SDPA form here")
+    n, m = inst.n, inst.m
+    with open(path, "w") as f:
+        f.write(f"* {comment or inst.name}\n")
+        f.write(f"{m}\n1\n{n}\n")
+        f.write(" ".join(f"{v:.17g}" for v in inst.b) + "\n")
+        for k in range(m + 1):
+            lo, hi = int(inst.beg[k]), int(inst.beg[k + 1])
+            if hi == lo:
+                continue
+            jj, ii = _unpack(n, inst.idx[lo:hi].astype(np.int64))
+            vals = inst.elem[lo:hi] * (-1.0 if k == 0 else 1.0)
+            for j, i, v in zip(jj, ii, vals):
+                f.write(f"{k} 1 {j + 1} {i + 1} {v:.17g}\n")
+
+
+def read_sdpa(path: str, lib, name: str | None = None) -> SDPInstance:
+    """One SDP block read by the library's ``HReadSDPA`` (include/hdsdp_file_io.h)."""
+    import ctypes as C
+
+    P_INT, P_DBL = C.POINTER(C.c_int), C.POINTER(C.c_double)
+    m, nb, ncols, nlp, nel = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    dims, rhs = P_INT(), P_DBL()
+    cbeg, cidx, celem = C.POINTER(P_INT)(), C.POINTER(P_INT)(), C.POINTER(P_DBL)()
+    lbeg, lidx, lelem = P_INT(), P_INT(), P_DBL()
+    rc = lib.HReadSDPA(path.encode(), C.byref(m), C.byref(nb), C.byref(dims), C.byref(rhs), C.byref(cbeg),
+                       C.byref(cidx), C.byref(celem), C.byref(ncols), C.byref(nlp), C.byref(lbeg), C.byref(lidx),
+                       C.byref(lelem), C.byref(nel))
+    if rc != 0:
+        raise ValueError(f"HReadSDPA failed on {path} ({rc})")
+    try:
+        if nb.value != 1 or nlp.value != 0:
+            raise ValueError(f"{path}: {nb.value} SDP blocks and an LP block of {nlp.value}; one SDP block expected")
+        n, mm = dims[0], m.value
+        beg = np.ctypeslib.as_array(cbeg[0], shape=(mm + 2,)).astype(np.int64)
+        nnz = int(beg[-1])
+        idx = np.ctypeslib.as_array(cidx[0], shape=(max(nnz, 1),))[:nnz].astype(np.int32)
+        elem = np.ctypeslib.as_array(celem[0], shape=(max(nnz, 1),))[:nnz].astype(np.float64)
+        b = np.ctypeslib.as_array(rhs, shape=(mm,)).astype(np.float64)
+        return SDPInstance(name or path, n, mm, beg, idx.copy(), elem.copy(), b.copy())
+    finally:
+        lib.HFreeSDPA(nb.value, dims, rhs, cbeg, cidx, celem, lbeg, lidx, lelem)

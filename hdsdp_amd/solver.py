@@ -168,6 +168,13 @@ class Solver:
         return SolveResult(int(rc), int(st.value), int(it.value), y, pobj.value, dobj.value, mu.value,
                            secs.value, secs_gap.value, dimacs, log, prof)
 
+    def profile_now(self) -> dict:
+        """Device-time profile so far: slot -> (ms, timed regions); needs profile=True."""
+        ns = len(_capi.PROF_SLOTS)
+        ms, cnt = np.zeros(ns), np.zeros(ns)
+        self.lib.HDSDPGetProfile(self.h, _capi.dptr(ms), _capi.dptr(cnt), ns)
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(_capi.PROF_SLOTS)}
+
     def setup(self) -> None:
         """Pre-solve and device set-up without the interior-point iterations."""
         if self.lib.HDSDPSetup(self.h):

@@ -97,10 +97,13 @@ hdsdp_retcode hdsdp_dev_ratio_test( hdsdp_dev *dev, int whichBuffer, double *max
 
 /* ---- Schur complement (coneBuildSchur slot) ----------------------------- */
 
-/* Build with W = S^-1 from the dual factor (or W = X for PRIMAL):
+/* Build with W = S^-1 from the dual factor:
  *   M_ij = <A_i, W A_j W> (lower triangle, not for CORRECTOR),
  *   dASinv[i] = <A_i, W>, dASinvRdSinv[i] = Rd <A_i, W W>,
  *   *dTraceSinv = tr(W) when Rd != 0 (not for CORRECTOR), else 0.
+ * typeKKT INFEASIBLE (0) and CORRECTOR (1) only: HOMOGENEOUS (2, the HSD
+ * scalars) and PRIMAL (3, W = X) return HDSDP_RETCODE_FAILED; the host runs
+ * without the HSD phase and with INT_PARAM_PSDP = 0.
  * Replaces sdpDenseConeGetKKT and its strategies M2-M5
  * (interface/hdsdp_conic_sdp.c:683-1052, 1722-1804). */
 hdsdp_retcode hdsdp_dev_schur( hdsdp_dev *dev, int typeKKT, double dualResidual,
@@ -152,7 +155,10 @@ hdsdp_retcode hdsdp_dev_kkt_solve_exact( hdsdp_dev *dev, const double *rhs, doub
 
 /* Differentiates the barrier optimality conditions b = mu A(S^-1),
  * S = C - sum_i y_i A_i, at the current dual factor and Schur matrix:
- * solves M lambda = dLdy exactly, then dLdb = lambda / mu and
+ * solves M lambda = dLdy exactly -- reusing the FP64 factor of the Schur
+ * matrix last factored on the device (the final centring step's) as the
+ * preconditioner of iterative refinement on the current M when it is close
+ * enough, factoring M otherwise -- then dLdb = lambda / mu and
  * dLdC = S^-1 (sum_i lambda_i A_i) S^-1 (n x n full).  In the form
  * -mu S^-1 (sum_i l_i A_i) S^-1 this is l = -lambda / mu.
  * Callers: HDSDPAdjoint (autograd through the solver, hdsdp_amd.adjoint). */
@@ -162,7 +168,9 @@ hdsdp_retcode hdsdp_dev_adjoint( hdsdp_dev *dev, double mu, const double *dLdy,
 /* ---- statistics ---------------------------------------------------------- */
 
 /* Counters for measurement: [0] Schur builds, [1] Cholesky of S, [2] CG
- * iterations, [3] Lanczos matvecs, [4] Cholesky of M, [5] last CG iterations. */
+ * iterations, [3] Lanczos matvecs, [4] Cholesky of M, [5] last CG iterations,
+ * [6] adjoint solves that refined with the factor of an earlier M instead of
+ * factoring again. */
 hdsdp_retcode hdsdp_dev_stats( hdsdp_dev *dev, double *stats, int nStats );
 
 /* Device-time profile, measured with events on the library's own stream.
@@ -176,6 +184,8 @@ hdsdp_retcode hdsdp_dev_stats( hdsdp_dev *dev, double *stats, int nStats );
  *   HDSDP_PROF_LANCZOS   one Lanczos step-length call
  *   HDSDP_PROF_BUFFER    one dual-buffer assembly
  *   HDSDP_PROF_PRIMAL    primal recovery
+ *   HDSDP_PROF_GRAM      the Gram products of a Schur build (dense: T_p = L^-1 A_p L^-T
+ *                        and <T_p, T_q>; rank-one: W F and F^T W F), FP64 or bf16 split
  * Profiling is off by default: enabling it adds one event synchronisation per
  * timed region.  No counterpart in the reference (measurement only). */
 #define HDSDP_PROF_SCHUR   0
@@ -186,7 +196,8 @@ hdsdp_retcode hdsdp_dev_stats( hdsdp_dev *dev, double *stats, int nStats );
 #define HDSDP_PROF_LANCZOS 5
 #define HDSDP_PROF_BUFFER  6
 #define HDSDP_PROF_PRIMAL  7
-#define HDSDP_PROF_SLOTS   8
+#define HDSDP_PROF_GRAM    8
+#define HDSDP_PROF_SLOTS   9
 hdsdp_retcode hdsdp_dev_set_profiling( hdsdp_dev *dev, int enable );
 int hdsdp_dev_get_profile( hdsdp_dev *dev, double *ms, double *counts, int nSlots );
 

@@ -9,11 +9,15 @@
  * Instance file layout: see hdsdp_amd/instances.py (write_instance).
  *
  * usage: hdsdp_ref_run INSTANCE YOUT [key=value ...]
- *   psdp=0|1 relopt=X absopt=X maxiter=K timelimit=T corra=K corrb=K
+ *   psdp=0|1 relopt=X absopt=X maxiter=K timelimit=T threads=K corra=K corrb=K
  *
- * The public setters reject the PSDP switch (interface/hdsdp.c:596-604), so it
- * is written through the solver struct (interface/def_hdsdp.h:131-134) after
- * HDSDPInit, which the solver never resets (interface/hdsdp.c:130-376).
+ * The public setters reject the PSDP switch and the thread count
+ * (interface/hdsdp.c:596-604: indices 3 and 4 are dropped), so both are
+ * written through the solver struct (interface/def_hdsdp.h:131-134) after
+ * HDSDPInit, which the solver never resets (interface/hdsdp.c:130-376).  The
+ * pre-solve then caps the thread count at what MKL reports
+ * (interface/hdsdp.c:306-315) and prints the count it uses ("Optimizing over N
+ * thread(s)", interface/hdsdp.c:669).
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -90,7 +94,7 @@ int main(int argc, char **argv) {
     if (absopt > 0) HDSDPSetDblParam(solver, DBL_PARAM_ABSOPTTOL, absopt);
     if (maxiter > 0) HDSDPSetIntParam(solver, INT_PARAM_MAXITER, maxiter);
     if (timelimit > 0) HDSDPSetDblParam(solver, DBL_PARAM_TIMELIMIT, timelimit);
-    if (threads > 0) HDSDPSetIntParam(solver, INT_PARAM_THREADS, threads);
+    if (threads > 0) solver->HIntParams[INT_PARAM_THREADS] = threads;
     (void) corra; (void) corrb;
     rc = HDSDPOptimize(solver, 1);
     double t1 = now_s();
@@ -102,9 +106,9 @@ int main(int argc, char **argv) {
         if (fy) { fwrite(y, sizeof(double), m, fy); fclose(fy); }
         fflush(stdout);
         printf("\nRESULT {\"retcode\": %d, \"status\": %d, \"iters\": %d, \"pobj\": %.17g, \"dobj\": %.17g, "
-               "\"mu\": %.17g, \"time\": %.6f, \"dimacs\": [%.17g, %.17g, %.17g, %.17g, %.17g, %.17g]}\n",
+               "\"mu\": %.17g, \"time\": %.6f, \"threads\": %d, \"dimacs\": [%.17g, %.17g, %.17g, %.17g, %.17g, %.17g]}\n",
                rc, (int) solver->HStatus, solver->nIterCount, pobj, dobj, solver->dBarrierMu, t1 - t0,
-               solver->dErrs[0], solver->dErrs[1], solver->dErrs[2], solver->dErrs[3], solver->dErrs[4],
+               solver->HIntParams[INT_PARAM_THREADS], solver->dErrs[0], solver->dErrs[1], solver->dErrs[2], solver->dErrs[3], solver->dErrs[4],
                solver->dErrs[5]);
         fflush(stdout);
         free(y);

@@ -280,10 +280,12 @@ def load_golden(name):
 
     with open(os.path.join(GOLDEN, name + ".json")) as f:
         fix = json.load(f)
-    make, opts = mg.CASES[name]
+    make, opts, _ = mg.CASES[name]
     inst = make()
     assert mg.instance_digest(inst) == fix["digest"], "instance generator drifted from the fixture"
     fix["y"] = np.load(os.path.join(GOLDEN, name + "_y.npy"))
+    if "dense" in fix:
+        fix["dense"]["y"] = np.load(os.path.join(GOLDEN, name + "_dense_y.npy"))
     return inst, fix
 
 
@@ -306,6 +308,61 @@ def parity_report(fix, res):
         "pobj_rel": abs(res.pobj - fix["pobj"]) / max(1.0, abs(fix["pobj"])),
         "y_rel": float(np.linalg.norm(res.y - y0) / max(np.linalg.norm(y0), 1e-300)),
     }
+
+
+# The parity bar: 1e-8 relative, or the reference's own spread on the instance where that
+# is wider (the largest distance between reference runs that differ only in the order of
+# their arithmetic: thread count, MKL instruction-set path, PARDISO or LAPACK for the
+# dual matrix -- recorded in each fixture by make_golden.py).
+PARITY_FLOOR = 1e-8
+
+
+def golden_tolerances(fix):
+    sp = fix["spread"]
+    return {k: max(PARITY_FLOOR, sp[k]) for k in ("y", "pobj", "dobj")}
+
+
+def check_dimacs(fix, res):
+    """The six DIMACS errors of a solve against the reference's on the same instance
+    (interface/hdsdp.c:753-906): errors 2-4 (lambda_max(X) sign, dual infeasibility, the
+    fixed zero) equal, error 1 (||AX - b||) at rounding level, and errors 5 and 6 (gap and
+    <X, S> over |pObj| + |dObj| + 1) within the objectives' parity tolerances, which bound
+    how far the final iterate may sit from the reference's."""
+    tol = golden_tolerances(fix)
+    ref, got = np.asarray(fix["dimacs"]), np.asarray(res.dimacs)
+    print(fix["name"], "dimacs ref", ref, "got", got, flush=True)
+    assert got[1] == ref[1] and got[2] == ref[2] and got[3] == ref[3]
+    assert got[0] <= max(1e-12, 10.0 * ref[0])
+    for k in (4, 5):
+        assert abs(got[k] - ref[k]) <= tol["pobj"] + tol["dobj"], (k, got[k], ref[k])
+
+
+def check_parity(fix, res, label=""):
+    """Status and iteration count equal to the reference's default build (and to its
+    dense-dual build where the fixture has one, unless the reference's own spread runs
+    reach this count as well); y*, pObj and dObj within golden_tolerances of both.
+    Returns the reports."""
+    tol = golden_tolerances(fix)
+    reps = {"default": parity_report(fix, res)}
+    if "dense" in fix:
+        reps["dense"] = parity_report(fix["dense"], res)
+    print(label or fix["name"], {"tol": tol, **reps}, flush=True)
+    # iteration counts the reference itself reaches on this input (its own spread runs)
+    ref_iters = {fix["iters"]} | {r["iters"] for r in fix["spread"]["runs"]}
+    for which, rep in reps.items():
+        ref = fix if which == "default" else fix["dense"]
+        assert res.status == ref["status"], (which, rep)
+        if which == "default":
+            assert res.iters == ref["iters"], (which, rep)
+        else:
+            assert res.iters == ref["iters"] or res.iters in ref_iters, (which, rep, sorted(ref_iters))
+        assert rep["dobj_rel"] <= tol["dobj"], (which, rep, tol)
+        assert rep["pobj_rel"] <= tol["pobj"], (which, rep, tol)
+        assert rep["y_rel"] <= tol["y"], (which, rep, tol)
+    ref = [(int(r[0]), int(r[1])) for r in fix["log"]]
+    got = [(int(r[0]), int(r[1])) for r in res.log]
+    assert got == ref
+    return reps
 
 
 def adjoint_fd_check(lib, inst, n_dirs, seed=0, h=1e-6, relopt=1e-8, **solve_opts):

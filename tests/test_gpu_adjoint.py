@@ -29,15 +29,19 @@ from hdsdp_amd.solver import Solver
 pytestmark = pytest.mark.gpu
 
 FD_CASES = {
-    "maxcut_n200": (lambda: I.maxcut(200, 0.05, 5, "unif"), 6),
-    "dense_n40_m60": (lambda: I.dense_random(40, 60, 24), 4),
+    # name: (instance, directions, gap of the solve)
+    "maxcut_n200": (lambda: I.maxcut(200, 0.05, 5, "unif"), 6, 1e-7),
+    "dense_n40_m60": (lambda: I.dense_random(40, 60, 24), 4, 1e-7),
+    # the adjoint workload at its own size: MaxCut on G(2000, 0.01), weights U(0, 1),
+    # solved to a 1e-8 gap, 20 seeded directions (measured max 3.3e-6, median 7.5e-7)
+    "maxcut_n2000_gap1e-8": (lambda: I.maxcut(2000, 0.01, 5, "unif"), 20, 1e-8),
 }
 
 
 @pytest.mark.parametrize("name", list(FD_CASES))
 def test_adjoint_matches_central_fd(gpu_lib, name):
-    make, ndir = FD_CASES[name]
-    errs, mu, db, dC, y = dt.adjoint_fd_check(gpu_lib, make(), ndir, relopt=1e-7)
+    make, ndir, gap = FD_CASES[name]
+    errs, mu, db, dC, y = dt.adjoint_fd_check(gpu_lib, make(), ndir, relopt=gap)
     print(name, "mu", mu, "rel errs", errs, flush=True)
     assert max(errs) < 1e-5
     assert np.array_equal(dC, dC.T)
@@ -58,6 +62,33 @@ def test_adjoint_matches_oracle_at_same_point(gpu_lib, oracle, name):
     e_b = np.linalg.norm(db - db0) / np.linalg.norm(db0)
     e_C = np.linalg.norm(dC - dC0) / np.linalg.norm(dC0)
     print(name, "mu", mu, "dLdb rel", e_b, "dLdC rel", e_C, flush=True)
+    assert e_b < 1e-6 and e_C < 1e-6
+
+
+def test_adjoint_reuses_the_centring_factor(gpu_lib, oracle):
+    """HDSDPAdjoint right after the centring does not factor the Schur matrix again: the
+    last Newton step's FP64 factor (of the Schur matrix one step before the centred point)
+    preconditions iterative refinement on the centred point's M.  No new factorisation is
+    counted, and the gradients match the oracle's (which factors afresh) within 1e-6."""
+    inst = I.maxcut(60, 0.1, 7, "unif")
+    g = np.random.default_rng(3).standard_normal(inst.m)
+    sv = Solver(gpu_lib, inst, relopt=1e-8, absopt=1e30, profile=True)
+    try:
+        sv.optimize()
+        mu, y0 = sv.barrier_point()
+        y, _, _ = sv.center(mu, y0)
+        before = sv.profile_now()["mfactor"][1]
+        db, dC = sv.adjoint(mu, y, g)
+        after = sv.profile_now()["mfactor"][1]
+    finally:
+        sv.close()
+    with Solver(oracle, inst) as so:
+        so.setup()
+        db0, dC0 = so.adjoint(mu, y, g)
+    e_b = np.linalg.norm(db - db0) / np.linalg.norm(db0)
+    e_C = np.linalg.norm(dC - dC0) / np.linalg.norm(dC0)
+    print("factorisations before/after the adjoint", before, after, "dLdb rel", e_b, "dLdC rel", e_C, flush=True)
+    assert after == before
     assert e_b < 1e-6 and e_C < 1e-6
 
 

@@ -185,6 +185,35 @@ def test_schur_mixed_precision_gram(gpu_lib, name, monkeypatch):
     assert rel(a, [np.sum(Ai * W) for Ai in A]) < 1e-10
 
 
+@pytest.mark.parametrize("n,m", [(260, 300), (130, 700), (37, 90)])
+def test_dense_schur_mixed_planes(gpu_lib, n, m):
+    """The mixed-precision dense Schur matrix from bf16 planes (L^-T and A_p split by the
+    producing kernels, V_p = A_p L^-T written as planes by the product's epilogue, the
+    packed T_p padded to a multiple of 32 with zeros): several 128-wide tiles with ragged
+    edges (n = 260), the split-K Gram in parts plus a ragged last part (m = 700), and n not
+    a multiple of 8 (n = 37, padded planes).  Within 1e-6 relative of the FP64 closed form
+    and within 1e-6 of the FP64 device build."""
+    rng = np.random.default_rng(19)
+    coeffs = dt.rand_coeffs(rng, n, m, [dt.DENSE])
+    y, eye = dt.interior_point(coeffs, rng)
+    out = []
+    for prec in (1, 0):
+        d = dt.Dev(gpu_lib, coeffs, precision=prec)
+        d.update(1.0, -1.0, y, eye, 0)
+        assert d.factorize(0) == 1
+        d.schur(0, 0.0)
+        out.append(d.get_schur())
+        d.close()
+    Mx, M64 = out
+    W = np.linalg.inv(dt.dense_buffer(coeffs, 1.0, -1.0, y, eye))
+    L = np.linalg.cholesky(W)
+    T = np.stack([(L.T @ c.dense() @ L).ravel() for c in coeffs[1:]])
+    Mref = T @ T.T
+    print(n, m, "mixed vs closed form", rel(Mx, Mref), "vs fp64", rel(Mx, M64), flush=True)
+    assert rel(Mx, Mref) < 1e-6
+    assert rel(Mx, M64) < 1e-6
+
+
 @pytest.mark.parametrize("n,m", [(260, 300), (130, 700)])
 def test_dense_schur_large_tiles(gpu_lib, n, m):
     """Dense Schur matrices large enough for the 128-wide product kernels: the

@@ -28,6 +28,7 @@ y* within 1e-7, except on two rank-one instances, bound 1e-6:
 import numpy as np
 import pytest
 
+import devtools as dt
 from hdsdp_amd import instances as I
 from hdsdp_amd.solver import run_solver
 
@@ -65,3 +66,51 @@ def test_mixed_matches_fp64(gpu_lib, name, monkeypatch):
     assert abs(rmx.dobj - r64.dobj) <= 1e-8 * max(1.0, abs(r64.dobj))
     assert abs(rmx.pobj - r64.pobj) <= 1e-8 * max(1.0, abs(r64.pobj))
     assert rel_y <= ytol
+
+
+# The two mixed-precision workloads at their own sizes: the dense random SDP n = 1000,
+# m = 2000 (every Gram block from bf16 planes) and the rank-one SDP n = 2000, m = 20000
+# in factor form (FP64 Gram, FP32 Cholesky preconditioner, lower-triangle CG products).
+FULL = {
+    "dense_n1000_m2000": (lambda: I.dense_random(1000, 2000, 3), True),
+    "rankone_n2000_m20000": (lambda: I.rank_one(2000, 20000, 20, 4), False),
+}
+
+
+@pytest.mark.parametrize("name", list(FULL))
+def test_mixed_full_size(gpu_lib, name):
+    """Mixed against FP64 at full size, 1e-6 gap: same status, at most two iterations
+    apart, objectives within 1e-8 and y* within 1e-7 relative."""
+    make, gram = FULL[name]
+    inst = make()
+    opts = dict(relopt=1e-6, absopt=1e30)
+    print(f"[mixed-full] {name}: fp64 solve", flush=True)
+    r64 = run_solver(gpu_lib, inst, precision=0, **opts)
+    print(f"[mixed-full] {name}: mixed solve", flush=True)
+    rmx = run_solver(gpu_lib, inst, precision=1, **opts)
+    rel_y = np.linalg.norm(rmx.y - r64.y) / np.linalg.norm(r64.y)
+    print(name, "fp64", r64.status, r64.iters, f"{r64.seconds:.2f}s", "mixed", rmx.status, rmx.iters,
+          f"{rmx.seconds:.2f}s", "y", rel_y, "gap", rmx.rel_gap, flush=True)
+    assert rmx.status == r64.status == 3
+    assert abs(rmx.iters - r64.iters) <= 2
+    assert abs(rmx.dobj - r64.dobj) <= 1e-8 * max(1.0, abs(r64.dobj))
+    assert abs(rmx.pobj - r64.pobj) <= 1e-8 * max(1.0, abs(r64.pobj))
+    assert rel_y <= 1e-7
+
+
+@pytest.mark.parametrize("name", ["dense_n200_m400", "rankone_n200_m2000", "dense_n120_m240"])
+def test_mixed_against_reference(gpu_lib, name, monkeypatch):
+    """The mixed path (bf16 planes for every dense / rank-one Gram block, forced for
+    rank-one) against the reference HDSDP's FP64 result on the same instance: same status,
+    at most two iterations apart, objectives within 1e-8 or the reference's own spread,
+    y* within 1e-7 or that spread."""
+    monkeypatch.setenv("HDSDP_GRAM_BF16", "1")
+    inst, fix = dt.load_golden(name)
+    res = run_solver(gpu_lib, inst, psdp=0, precision=1, **fix["options"])
+    rep = dt.parity_report(fix, res)
+    tol = dt.golden_tolerances(fix)
+    print(name, rep, tol, flush=True)
+    assert res.status == fix["status"]
+    assert abs(res.iters - fix["iters"]) <= 2
+    assert rep["dobj_rel"] <= tol["dobj"] and rep["pobj_rel"] <= tol["pobj"]
+    assert rep["y_rel"] <= max(1e-7, tol["y"])

@@ -1,19 +1,20 @@
 """End-to-end parity of the HIP solver against the reference HDSDP build.
 
 Each golden fixture holds the reference's status, iteration count, objectives
-and y* for one seeded instance (written by tests/golden/make_golden.py).  The
-plain FP64 GPU path must reproduce status and iteration count exactly, the
-objectives within 1e-8 relative, and y* within the tolerance listed per case:
-y* is determined only to the accuracy the stopping rule leaves (directions
-along which the dual objective is flat move freely once the gap is below
-the target), so for runs stopped at a 1e-6 gap y* is compared at 1e-5.
-Intermediate iterates are not compared value by value: the reference's
-Lanczos step-length routine (linalg/hdsdp_lanczos.c:176-292) runs without
-reorthogonalisation, so once a Ritz value converges a change in summation
-order grows into a different (equally valid) step bound; the phase sequence
-of the log is compared instead.  The small cases also run the CPU oracle
-backend on the same host code, which must agree with the GPU to the same
-objective tolerances.
+and y* for one seeded instance (written by tests/golden/make_golden.py), and,
+for instances whose dual matrix the reference factorises with PARDISO, the
+result of the reference's dense-dual (LAPACK) build as well.  The plain FP64
+GPU path must reproduce status and iteration count exactly and y*, pObj and
+dObj within 1e-8 relative of both -- or within the reference's own spread on
+that instance where that is wider: the largest distance between reference runs
+that differ only in the order of their arithmetic (one thread or all, MKL's
+AVX-512 / AVX2 / compatible code paths, PARDISO or LAPACK), recorded in the
+fixture.  Intermediate iterates are not compared value by value: the
+reference's Lanczos step-length routine (linalg/hdsdp_lanczos.c:176-292) runs
+without reorthogonalisation, so once a Ritz value converges a change in
+summation order grows into a different (equally valid) step bound; the phase
+sequence of the log is compared instead.  The small cases also run the CPU
+oracle backend on the same host code, which must meet the same bar.
 """
 import numpy as np
 import pytest
@@ -24,47 +25,29 @@ from hdsdp_amd.solver import run_solver
 
 pytestmark = pytest.mark.gpu
 
-# name: (pobj tol, y tol, also run the oracle backend)
+# name: also run the oracle backend
 CASES = {
-    "theta_n50": (1e-8, 1e-6, True),
-    "maxcut_n60": (1e-8, 1e-6, True),
-    "dense_n20_m30": (1e-7, 1e-6, True),
-    "dense_n40_m60": (1e-8, 1e-6, True),
-    "c1_default": (1e-8, 1e-7, True),
-    "c1_gap1e-6": (1e-7, 1e-5, False),
-    "c2_gap1e-6": (1e-7, 1e-5, False),
-    "c5_gap1e-6": (1e-7, 1e-5, False),
-    "dense_n120_m240": (1e-7, 1e-5, False),
-    "rankone_n60_m500": (1e-7, 1e-5, False),
-    "dense_n200_m400": (1e-7, 1e-5, False),
-    "rankone_n200_m2000": (1e-7, 1e-5, False),
+    "theta_n50": True,
+    "maxcut_n60": True,
+    "dense_n20_m30": True,
+    "dense_n40_m60": True,
+    "c1_default": True,
+    "c1_gap1e-6": False,
+    "c2_gap1e-6": False,
+    "c5_gap1e-6": False,
+    "c5_gap1e-8": False,
+    "dense_n120_m240": False,
+    "rankone_n60_m500": False,
 }
-
-
-def check_against_golden(fix, res, ptol, ytol):
-    rep = dt.parity_report(fix, res)
-    print(fix["name"], rep, flush=True)
-    assert res.status == fix["status"]
-    assert res.iters == fix["iters"]
-    assert rep["dobj_rel"] <= 1e-8
-    assert rep["pobj_rel"] <= ptol
-    assert rep["y_rel"] <= ytol
-    # same log records: phase A iterations, then phase B iterations
-    ref = [(int(r[0]), int(r[1])) for r in fix["log"]]
-    got = [(int(r[0]), int(r[1])) for r in res.log]
-    assert got == ref
 
 
 @pytest.mark.parametrize("name", list(CASES))
 def test_solve_matches_reference(gpu_lib, oracle, name):
-    ptol, ytol, with_oracle = CASES[name]
     inst, fix, res = dt.solve_like_golden(gpu_lib, name)
-    check_against_golden(fix, res, ptol, ytol)
-    if with_oracle:
+    dt.check_parity(fix, res, "gpu " + name)
+    if CASES[name]:
         _, _, ores = dt.solve_like_golden(oracle, name)
-        assert ores.iters == res.iters and ores.status == res.status
-        assert abs(ores.dobj - res.dobj) <= 1e-8 * max(1.0, abs(res.dobj))
-        assert abs(ores.pobj - res.pobj) <= ptol * max(1.0, abs(res.pobj))
+        dt.check_parity(fix, ores, "oracle " + name)
 
 
 def test_rank_one_factor_form(gpu_lib):
@@ -75,14 +58,14 @@ def test_rank_one_factor_form(gpu_lib):
 
     _, fix = dt.load_golden("rankone_n60_m500")
     res = run_solver(gpu_lib, I.rank_one(60, 500, 5, 45), relopt=1e-6, absopt=1e30)
-    check_against_golden(fix, res, 1e-7, 1e-5)
+    dt.check_parity(fix, res, "gpu rankone_n60_m500 factor form")
 
 
-def test_solution_checks(gpu_lib):
-    """DIMACS errors of the GPU solution are as small as the reference's."""
-    inst, fix, res = dt.solve_like_golden(gpu_lib, "c1_default")
-    for got, ref in zip(res.dimacs, fix["dimacs"]):
-        assert got <= max(10 * abs(ref), 1e-9)
+@pytest.mark.parametrize("name", ["c1_default", "theta_n50", "dense_n40_m60", "c1_gap1e-6"])
+def test_solution_checks(gpu_lib, name):
+    """All six DIMACS errors of the GPU solution at the reference's values."""
+    inst, fix, res = dt.solve_like_golden(gpu_lib, name)
+    dt.check_dimacs(fix, res)
 
 
 def test_setup_then_optimize_is_optimize(gpu_lib):
