@@ -64,6 +64,8 @@ _API = {
     "hdsdp_dev_kkt_solve": (C.c_int, [P_VOID, P_DBL, P_DBL]),
     "hdsdp_dev_kkt_solve_exact": (C.c_int, [P_VOID, P_DBL, P_DBL]),
     "hdsdp_dev_get_schur": (C.c_int, [P_VOID, P_DBL]),
+    "hdsdp_dev_kkt_solve_noalloc": (C.c_int, [P_VOID, C.c_int, P_VOID, P_VOID, C.c_double, C.c_double, C.c_int,
+                                              C.c_int, P_VOID, P_INT]),
     "hdsdp_dev_primal": (C.c_int, [P_VOID, C.c_double, P_DBL, P_DBL, P_INT]),
     "hdsdp_dev_primal_stats": (C.c_int, [P_VOID, P_DBL, P_DBL, P_DBL, P_DBL]),
     "hdsdp_dev_get_primal": (C.c_int, [P_VOID, P_DBL]),

@@ -130,6 +130,23 @@ hdsdp_retcode hdsdp_dev_kkt_solve( hdsdp_dev *dev, const double *rhs, double *so
  * (interface/hdsdp_schur.c:293-326 exports only its vectors). */
 hdsdp_retcode hdsdp_dev_get_schur( hdsdp_dev *dev, double *M );
 
+/* Jacobi-preconditioned CG on a caller-held Schur matrix without any device
+ * allocation.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  M: m x m FP64, column-major, both
+ * triangles; rhs, sol: device m-vectors (may alias); stream: a hipStream_t or
+ * NULL.  The iteration is hdsdp_dev_kkt_solve's while it uses the Jacobi
+ * preconditioner (same kernels, same order: the results are identical); the
+ * CG vectors live in a static device array of the library (about 4 MiB per
+ * GPU, held until the process ends), so m <= 65536, and calls that overlap on
+ * one GPU race.  Returns HDSDP_RETCODE_FAILED for m outside 1..65536, and when
+ * the Jacobi iteration would switch to the Cholesky preconditioner (that needs
+ * m^2 device memory: use hdsdp_dev_kkt_solve).  A right-hand side below the
+ * tolerance leaves sol as it was, as conjGradSolve does.  *nIter: iterations. */
+hdsdp_retcode hdsdp_dev_kkt_solve_noalloc( const double *M, int m, const double *rhs, double *sol,
+                                           double relTol, double absTol, int maxIter, int nRestartFreq,
+                                           void *stream, int *nIter );
+
 /* ---- primal recovery and solution checks -------------------------------- */
 
 /* X = mu (S^-1 + S^-1 (sum dy_i A_i) S^-1) with S = C - A^T y (+ perturbation)

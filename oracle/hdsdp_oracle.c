@@ -750,6 +750,67 @@ hdsdp_retcode hdsdp_dev_kkt_solve( hdsdp_dev *d, const double *rhs, double *sol 
     return cg_solve(d, d->rhsbuf, sol);
 }
 
+/* hdsdp_dev_kkt_solve_noalloc on host memory: the Jacobi-preconditioned CG of
+ * cg_solve above (conjGradSolve, linalg/hdsdp_linsolver.c:855-997) on a caller-held
+ * full symmetric M, FAILED where cg_solve would switch to the Cholesky
+ * preconditioner.  The pointers are host pointers here. */
+hdsdp_retcode hdsdp_dev_kkt_solve_noalloc( const double *M, int m, const double *rhs, double *sol, double relTol,
+                                           double absTol, int maxIter, int nRestartFreq, void *stream, int *nIter ) {
+    (void) stream;
+    if ( nIter ) *nIter = 0;
+    if ( m <= 0 || m > 65536 || !M || !rhs || !sol || !(relTol > 0.0) || !(absTol > 0.0) || maxIter <= 0 )
+        return HDSDP_RETCODE_FAILED;
+    double *buf = (double *) calloc((size_t) 7 * m, sizeof(double));
+    if ( !buf ) return HDSDP_RETCODE_MEMORY;
+    double *jac = buf, *r = buf + m, *rn = buf + 2 * m, *p = buf + 3 * m, *z = buf + 4 * m, *Mp = buf + 5 * m;
+    double *x = buf + 6 * m;
+    hdsdp_retcode rc = HDSDP_RETCODE_FAILED;
+    int restart = OMAX(nRestartFreq, 20), iter = 0, converged = 0;
+    for ( int i = 0; i < m; ++i ) jac[i] = FE(M, m, i, i);
+    memcpy(r, rhs, sizeof(double) * m);
+    double rnorm = onrm2(m, r), bnorm = onrm2(m, rhs);
+    double tol = OMIN(absTol, bnorm * relTol);
+    tol = OMAX(tol, 0.1 * absTol);
+    if ( rnorm < tol ) {
+        rc = HDSDP_RETCODE_OK; /* solution left unwritten, as the reference */
+        goto done;
+    }
+    for ( int i = 0; i < m; ++i ) p[i] = z[i] = r[i] / jac[i];
+    osymv(m, 1.0, M, p, Mp);
+    for ( iter = 0; iter < maxIter; ++iter ) {
+        double rz = odot(m, z, r), pMp = odot(m, p, Mp), alpha = rz / pMp;
+        for ( int i = 0; i < m; ++i ) x[i] += alpha * p[i];
+        if ( iter % restart == 5 ) {
+            osymv(m, 1.0, M, x, p);
+            for ( int i = 0; i < m; ++i ) r[i] = rhs[i] - p[i];
+            for ( int i = 0; i < m; ++i ) p[i] = z[i] = r[i] / jac[i];
+            osymv(m, 1.0, M, p, Mp);
+            continue;
+        }
+        for ( int i = 0; i < m; ++i ) rn[i] = r[i] - alpha * Mp[i];
+        for ( int i = 0; i < m; ++i ) z[i] = rn[i] / jac[i];
+        double beta = odot(m, rn, z) / rz;
+        for ( int i = 0; i < m; ++i ) p[i] = z[i] + beta * p[i];
+        osymv(m, 1.0, M, p, Mp);
+        memcpy(r, rn, sizeof(double) * m);
+        rnorm = onrm2(m, r);
+        if ( rnorm != rnorm ) goto done;
+        if ( iter > 20 && rnorm > 0.01 * bnorm ) break;
+        if ( rnorm < tol ) {
+            converged = 1;
+            break;
+        }
+    }
+    if ( nIter ) *nIter = iter;
+    if ( converged ) {
+        memcpy(sol, x, sizeof(double) * m);
+        rc = HDSDP_RETCODE_OK;
+    }
+done:
+    free(buf);
+    return rc;
+}
+
 /* Direct FP64 solve of M x = rhs: Cholesky of M, then two steps of iterative
  * refinement on the residual with M. */
 hdsdp_retcode hdsdp_dev_kkt_solve_exact( hdsdp_dev *d, const double *rhs, double *sol ) {
