@@ -52,14 +52,15 @@ WORKLOADS = {
            ("the same instance, full solve to 1e-6 gap", lambda I: I.lovasz_theta(400, 0.1, 2), None)),
     "C3": ("dense random SDP, n=1000, m=2000, seed 3",
            lambda I: I.dense_random(1000, 2000, 3), None,
-           ("dense random SDP of the same family at n=200, m=400 (seed 34), first 5 IPM iterations "
-            "(the n=1000 instance is 12 GB of CSC input and minutes per reference iteration)",
-            lambda I: I.dense_random(200, 400, 34), 5)),
+           ("dense random SDP of the same family at n=120, m=240 (seed 33), full solve to 1e-6 gap "
+            "(the n=1000 instance is 12 GB of CSC input and minutes per reference iteration; the "
+            "reference aborts when an iteration cap ends its phase A, so no capped sample is taken)",
+            lambda I: I.dense_random(120, 240, 33), None)),
     "C4": ("rank-one SDP (factor form), n=2000, m=20000, r=20, seed 4",
            lambda I: I.rank_one(2000, 20000, 20, 4), None,
-           ("rank-one SDP of the same family at n=200, m=2000 (seed 4, CSC input), first 3 IPM iterations "
+           ("rank-one SDP of the same family at n=60, m=500 (seed 45, CSC input), full solve to 1e-6 gap "
             "(the n=2000, m=20000 instance overflows the reference's int32 CSC input)",
-            lambda I: I.rank_one(200, 2000, 20, 4, form="csc"), 3)),
+            lambda I: I.rank_one(60, 500, 5, 45, form="csc"), None)),
     "C5": ("MaxCut SDP, G(n=2000, p=0.01) seed 5, weights U(0,1): n=2000, m=2000",
            lambda I: I.maxcut(2000, 0.01, 5, "unif"), "c5_gap1e-6",
            ("the same instance, full solve to 1e-6 gap", lambda I: I.maxcut(2000, 0.01, 5, "unif"), None)),
@@ -370,7 +371,7 @@ def main():
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         cb = cpu_baseline(cpu_sample)
-        if cb and "error" not in cb and cpu_sample[2] is not None:
+        if cb and "error" not in cb and cpu_sample[1](I).n != n:
             cb["gpu_same_sample"] = gpu_on_sample(lib, cpu_sample, precs)
         line["cpu_baseline"] = cb
     if args.workload == "C5" and args.adjoint_dirs > 0 and "fp64" in summary:

@@ -71,6 +71,10 @@ _API = {
     "hdsdp_dev_get_primal": (C.c_int, [P_VOID, P_DBL]),
     "hdsdp_dev_get_dual": (C.c_int, [P_VOID, P_DBL]),
     "hdsdp_dev_adjoint": (C.c_int, [P_VOID, C.c_double, P_DBL, P_DBL, P_DBL]),
+    "hdsdp_dev_psdp_init": (C.c_int, [P_VOID]),
+    "hdsdp_dev_psdp_rhs": (C.c_int, [P_VOID, C.c_double, P_DBL, P_DBL]),
+    "hdsdp_dev_psdp_direction": (C.c_int, [P_VOID, C.c_double, C.c_int, P_DBL]),
+    "hdsdp_dev_psdp_step": (C.c_int, [P_VOID, C.c_double, P_INT, P_DBL, P_DBL]),
     "hdsdp_dev_stats": (C.c_int, [P_VOID, P_DBL, C.c_int]),
     "hdsdp_dev_set_profiling": (C.c_int, [P_VOID, C.c_int]),
     "hdsdp_dev_get_profile": (C.c_int, [P_VOID, P_DBL, P_DBL, C.c_int]),

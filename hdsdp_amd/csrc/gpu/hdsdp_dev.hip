@@ -31,6 +31,8 @@ struct hdsdp_dev {
     std::vector<void *> allocs;
     DevCoeffs c;
     int cType;
+    int conDenseKind;  // a constraint coefficient is DENSE or DSR1
+    int psdpReady;     // hdsdp_dev_psdp_init has set the primal iterate
     std::vector<int> heavy;  // coefficient indices handled by the dense sandwich path
     int hasZero;
     // Gram formulations (all constraints of one kind): dense rank-one M = s s^T o (F^T W F)^2,
@@ -74,9 +76,14 @@ struct hdsdp_dev {
     int *hinfo;
     bool stagingBusy;
 
-    // Lanczos (linalg/hdsdp_lanczos.c)
-    int lzMax, lzComputed;
-    double *V, *Hd, *lv, *lw, *lz1, *lz2, *lwarm, *ly, *mv1, *mv2;
+    // Lanczos (linalg/hdsdp_lanczos.c): the dual cone's solver (lzComputed, lwarm) and the
+    // primal method's own (lzComputedP, lwarmP; interface/hdsdp_psdp.c:113-116)
+    int lzMax, lzComputed, lzComputedP;
+    double *V, *Hd, *lv, *lw, *lz1, *lz2, *lwarm, *lwarmP, *ly, *mv1, *mv2;
+    // primal method (interface/hdsdp_psdp.c): the primal iterate is X; Xs its value at the
+    // start (the fixed scaling), dXp the direction, XSX the buffer X S X (+ Xs dS Xs), Sold
+    // the dual matrix of the iteration, LX / LXi / LXiT the Cholesky factor of X and inverses
+    double *Xs, *dXp, *XSX, *Sold, *LX, *LXi, *LXiT;
     double *lzHist;  // (alpha_k, ||w_k||) of every Lanczos step, read at the check steps
     double *symvW;   // tile partials of the lower-triangle product with M (large m), else null
     std::vector<double> Hh;
@@ -90,6 +97,7 @@ struct hdsdp_dev {
     double *jac, *cr, *crn, *cp, *cz, *cMp, *cx, *crhs, *ctmp;
     double *precS;     // Jacobi scaling of the FP32 preconditioner
     double precShift;  // diagonal shift the FP32 factorisation needed
+    double refineFactor;  // mixed: the exact-operator solves stop at refineFactor * tol
     int precKind;
     int useJacobi, paramUseJacobi, maxIter, nRestart;
     double absTol, relTol;
@@ -203,12 +211,20 @@ hdsdp_retcode ensure_linv(hdsdp_dev *d, int f) {
     return check_launch();
 }
 
-// out = L^-1 (-dS) L^-T in for the factor f (interface/hdsdp_conic_sdp.c:462-505)
-void lz_mvec(hdsdp_dev *d, int f, const double *in, double *out) {
+// The Lanczos operator x -> L^-1 (-D) L^-T x: the dual cone's with the factor of a dual
+// buffer and D = dS (interface/hdsdp_conic_sdp.c:462-505), the primal method's with the
+// factor of X and D = the primal direction (interface/hdsdp_psdp.c:58-66).
+struct LzOp {
+    const double *Li, *LiT, *D;  // L^-1, L^-T, D (n x n, both triangles)
+    int *computed;               // calls so far (0: the fixed start vector)
+    double *warm;                // warm start of the next call
+};
+
+void lz_mvec(hdsdp_dev *d, const LzOp &op, const double *in, double *out) {
     const int n = d->n;
-    hdk::coldot<double>(d->st, n, n, 1, 1.0, d->Li[f], n, in, 0.0, d->mv1);
-    hdk::coldot<double>(d->st, n, n, 0, -1.0, d->S[2], n, d->mv1, 0.0, d->mv2);
-    hdk::coldot<double>(d->st, n, n, 2, 1.0, d->LiT[f], n, d->mv2, 0.0, out);
+    hdk::coldot<double>(d->st, n, n, 1, 1.0, op.Li, n, in, 0.0, d->mv1);
+    hdk::coldot<double>(d->st, n, n, 0, -1.0, op.D, n, d->mv1, 0.0, d->mv2);
+    hdk::coldot<double>(d->st, n, n, 2, 1.0, op.LiT, n, d->mv2, 0.0, out);
     d->stats[3] += 1;
 }
 
@@ -249,14 +265,14 @@ void lz_perturb(hdsdp_dev *d, int n, double *vVec) {
 // A breakdown or a converged check inside a window has let the device run the rest
 // of the window's steps; those only write Krylov columns beyond the ones the
 // estimate combines, so the result is that of the step-by-step loop.
-hdsdp_retcode lanczos(hdsdp_dev *d, int f, double *maxStep) {
+hdsdp_retcode lanczos(hdsdp_dev *d, const LzOp &op, double *maxStep) {
     const int n = d->n, maxDim = d->lzMax, nH = maxDim + 1;
     std::vector<double> &H = d->Hh;
     std::fill(H.begin(), H.end(), 0.0);
-    if (d->lzComputed == 0) {
+    if (*op.computed == 0) {
         lz_prepare(d, n, d->hvec_n);
     } else {
-        HIP_OK(hipMemcpyAsync(d->hvec_n, d->lwarm, n * sizeof(double), hipMemcpyDeviceToHost, d->st));
+        HIP_OK(hipMemcpyAsync(d->hvec_n, op.warm, n * sizeof(double), hipMemcpyDeviceToHost, d->st));
         DCALL(sync(d));
         lz_perturb(d, n, d->hvec_n);
     }
@@ -269,7 +285,7 @@ hdsdp_retcode lanczos(hdsdp_dev *d, int f, double *maxStep) {
     int kRead = 0;  // first step whose scalars the host has not processed
     bool done = false;
     for (int k = 0; k < maxDim && !done; ++k) {
-        lz_mvec(d, f, d->lv, d->lw);
+        lz_mvec(d, op, d->lv, d->lw);
         hdk::lz_step(d->st, n, k, d->lw, d->V, d->Hd, nH, d->lv, d->sc_d, d->lzHist);
         if ((k + 1) % check != 0 && k < maxDim - 1) continue;
         HIP_OK(hipMemcpyAsync(d->hHist + 2 * kRead, d->lzHist + 2 * kRead, sizeof(double) * 2 * (k + 1 - kRead),
@@ -292,11 +308,11 @@ hdsdp_retcode lanczos(hdsdp_dev *d, int f, double *maxStep) {
                 memcpy(d->hY, Y, sizeof(double) * 2 * kp1);
                 HIP_OK(hipMemcpyAsync(d->ly, d->hY, sizeof(double) * 2 * kp1, hipMemcpyHostToDevice, d->st));
                 hdk::lz_comb(d->st, n, kp1, d->V, d->ly + kp1, d->lz1);
-                lz_mvec(d, f, d->lz1, d->lz2);
-                HIP_OK(hipMemcpyAsync(d->lwarm, d->lz2, n * sizeof(double), hipMemcpyDeviceToDevice, d->st));
+                lz_mvec(d, op, d->lz1, d->lz2);
+                HIP_OK(hipMemcpyAsync(op.warm, d->lz2, n * sizeof(double), hipMemcpyDeviceToDevice, d->st));
                 hdk::axpy_norm(d->st, n, -e1, d->lz1, d->lz2, d->sc_d, hdk::LZ_R1);
                 hdk::lz_comb(d->st, n, kp1, d->V, d->ly, d->lz2);
-                lz_mvec(d, f, d->lz2, d->lz1);
+                lz_mvec(d, op, d->lz2, d->lz1);
                 hdk::axpy_norm(d->st, n, -e1, d->lz2, d->lz1, d->sc_d, hdk::LZ_R2);
                 DCALL(read_scalars(d));
                 double r1 = d->hsc[hdk::LZ_R1], r2 = d->hsc[hdk::LZ_R2];
@@ -315,7 +331,7 @@ hdsdp_retcode lanczos(hdsdp_dev *d, int f, double *maxStep) {
         }
         kRead = k + 1;
     }
-    d->lzComputed += 1;
+    *op.computed += 1;
     return check_launch();
 }
 
@@ -470,7 +486,7 @@ int cg_solve(hdsdp_dev *d) {
     double tol = std::min(d->absTol, bnorm * d->relTol);
     tol = std::max(tol, 0.1 * d->absTol);
     if (rnorm < tol) return 1;
-    const double target = exactOp ? 1e-4 * tol : tol;
+    const double target = exactOp ? d->refineFactor * tol : tol;
     apply_precond(d, d->cr, d->cz, d->cp);
     op(d->cp, d->cMp);
     int iter = 0, status = 0, refine = 0;
@@ -755,7 +771,9 @@ hdsdp_retcode hdsdp_dev_create(hdsdp_dev **pdev, int nRow, int nCol, int precisi
     DCALL(halloc(&d->hY, 64));
     DCALL(halloc(&d->hinfo, 1));
     d->lzMax = 30;
-    d->lzComputed = 0;
+    d->lzComputed = d->lzComputedP = 0;
+    d->psdpReady = 0;
+    d->lwarmP = d->Xs = d->dXp = d->XSX = d->Sold = d->LX = d->LXi = d->LXiT = nullptr;
     d->Hh.assign((size_t) (d->lzMax + 1) * (d->lzMax + 1), 0.0);
     DCALL(halloc(&d->hHist, 2 * (d->lzMax + 1)));
     DCALL(dalloc(d, &d->lzHist, 2 * (d->lzMax + 1)));
@@ -789,6 +807,8 @@ hdsdp_retcode hdsdp_dev_create(hdsdp_dev **pdev, int nRow, int nCol, int precisi
     d->absTol = 1e-06;
     d->relTol = 1e-06;
     d->useJacobi = d->paramUseJacobi = 1;
+    d->refineFactor = 1e-4;
+    if (const char *rf = getenv("HDSDP_MIXED_REFINE")) d->refineFactor = atof(rf);
     d->mVer = 0;
     d->cholVer = -1;
     d->maxIter = std::max(50, nRow / 20);
@@ -814,6 +834,9 @@ hdsdp_retcode hdsdp_dev_load(hdsdp_dev *d, const hdsdp_coeff_view *cv) {
     std::vector<int> cRow, cCol, denseCon, rCon;
     std::vector<double> cVal;
     d->cType = cv[0].type;
+    d->conDenseKind = 0;
+    for (int k = 1; k <= m; ++k)
+        if (cv[k].type == HDSDP_COEFF_DENSE || cv[k].type == HDSDP_COEFF_DSR1) d->conDenseKind = 1;
     d->hasZero = 0;
     for (int k = 0; k <= m; ++k) {
         ctype[k] = cv[k].type;
@@ -1019,9 +1042,10 @@ hdsdp_retcode hdsdp_dev_load(hdsdp_dev *d, const hdsdp_coeff_view *cv) {
         mixedGram = d->precision == HDSDP_PRECISION_MIXED && (d->r1Gram || d->dnGram) && atoi(force) != 0;
     d->schurMixed = mixedGram ? 1 : 0;
     d->mGramPending = 0;
-    // bf16 planes for the dense mixed Gram (split once by the producing kernels) or the
-    // split-in-the-loop products; HDSDP_BF16_PLANES=1 selects the planes
-    d->planes = 0;
+    // bf16 planes for the dense mixed Gram (split once by the producing kernels; 2.2x the
+    // split-in-the-loop products on the n = 1000 Gram); HDSDP_BF16_PLANES=0 selects the
+    // products that split their FP32 operands in the loop
+    d->planes = 1;
     if (const char *pl = getenv("HDSDP_BF16_PLANES")) d->planes = atoi(pl) != 0;
     if (mixedGram) {
         if (d->r1Gram || !d->planes) DCALL(dalloc(d, &d->LiTf, (size_t) n * n));
@@ -1244,7 +1268,8 @@ hdsdp_retcode hdsdp_dev_ratio_test(hdsdp_dev *d, int which, double *maxStep) {
     prof_begin(d, HDSDP_PROF_LANCZOS);
     DCALL(ensure_linv(d, f));
     hdk::mirror_lower<double>(d->st, d->n, d->S[2], d->n);
-    DCALL(lanczos(d, f, maxStep));
+    const LzOp op{d->Li[f], d->LiT[f], d->S[2], &d->lzComputed, d->lwarm};
+    DCALL(lanczos(d, op, maxStep));
     return prof_end(d, HDSDP_PROF_LANCZOS);
 }
 
@@ -1253,11 +1278,21 @@ hdsdp_retcode hdsdp_dev_ratio_test(hdsdp_dev *d, int which, double *maxStep) {
 hdsdp_retcode hdsdp_dev_schur(hdsdp_dev *d, int typeKKT, double Rd, double *asinv, double *asinvrd,
                               double *traceSinv) {
     const int m = d->m, n = d->n;
-    if (typeKKT == 2 || typeKKT == 3) return HDSDP_RETCODE_FAILED;
-    if (typeKKT != 1) prof_begin(d, HDSDP_PROF_SCHUR);
-    DCALL(ensure_linv(d, 0));
-    // W = S^-1 = L^-T L^-1
-    hdk::gemm<double>(d->st, true, false, false, n, n, n, 1.0, d->Li[0], n, 0, d->Li[0], n, 0, 0.0, d->W, n, 0, 1);
+    if (typeKKT == 2) return HDSDP_RETCODE_FAILED;
+    if (typeKKT == 3) {
+        // PRIMAL: strategy M5 with W := X for every column (interface/hdsdp_conic_sdp.c:1741-1745,
+        // 1813-1820); the reference's M5 takes no DENSE or DSR1 coefficient
+        // (linalg/hdsdp_sdpdata.c:1961-1965, 1995-1999)
+        if (d->conDenseKind || !d->psdpReady) return HDSDP_RETCODE_FAILED;
+        prof_begin(d, HDSDP_PROF_SCHUR);
+        HIP_OK(hipMemcpyAsync(d->W, d->X, sizeof(double) * (size_t) n * n, hipMemcpyDeviceToDevice, d->st));
+    } else {
+        if (typeKKT != 1) prof_begin(d, HDSDP_PROF_SCHUR);
+        DCALL(ensure_linv(d, 0));
+        // W = S^-1 = L^-T L^-1
+        hdk::gemm<double>(d->st, true, false, false, n, n, n, 1.0, d->Li[0], n, 0, d->Li[0], n, 0, 0.0, d->W, n, 0,
+                          1);
+    }
     if (Rd != 0.0)
         hdk::gemm<double>(d->st, false, false, false, n, n, n, 1.0, d->W, n, 0, d->W, n, 0, 0.0, d->W2, n, 0, 1);
     hdk::traces(d->st, d->c, 1, m, d->W, Rd != 0.0 ? d->W2 : nullptr, Rd, d->vecA_d, d->vecB_d, false);
@@ -1487,6 +1522,110 @@ hdsdp_retcode hdsdp_dev_get_dual(hdsdp_dev *d, double *S) {
     hdk::mirror_lower<double>(d->st, n, d->T1, n);
     DCALL(sync(d));
     HIP_OK(hipMemcpy(S, d->T1, sizeof(double) * (size_t) n * n, hipMemcpyDeviceToHost));
+    return HDSDP_RETCODE_OK;
+}
+
+// ---- primal method (interface/hdsdp_psdp.c) ------------------------------------------------
+
+namespace {
+
+// Cholesky of X into LX with its inverse factors; *isPsd = 0 when X is not positive definite
+// (HFpLinsysNumeric / HFpLinsysPsdCheck on the dense solver, linalg/hdsdp_linsolver.c:525-560).
+hdsdp_retcode factor_x(hdsdp_dev *d, int *isPsd) {
+    const int n = d->n;
+    HIP_OK(hipMemcpyAsync(d->LX, d->X, sizeof(double) * (size_t) n * n, hipMemcpyDeviceToDevice, d->st));
+    if (!hdk::potrf<double>(d->st, n, d->LX, n, d->T1, std::max((size_t) n * n, hdk::POTRF_WORK), d->info_d, &d->la))
+        return HDSDP_RETCODE_FAILED;
+    HIP_OK(hipMemcpyAsync(d->hinfo, d->info_d, sizeof(int), hipMemcpyDeviceToHost, d->st));
+    DCALL(sync(d));
+    *isPsd = (*d->hinfo == 0);
+    if (!*isPsd) return HDSDP_RETCODE_OK;
+    hdk::trtri<double>(d->st, n, d->LX, n, d->LXi, n, d->work);
+    hdk::transpose<double>(d->st, n, d->LXi, n, d->LXiT, n);
+    return check_launch();
+}
+
+}  // namespace
+
+hdsdp_retcode hdsdp_dev_psdp_init(hdsdp_dev *d) {
+    const int n = d->n;
+    const size_t nn = (size_t) n * n;
+    if (!d->Xs) {
+        DCALL(dalloc(d, &d->Xs, nn));
+        DCALL(dalloc(d, &d->dXp, nn));
+        DCALL(dalloc(d, &d->XSX, nn));
+        DCALL(dalloc(d, &d->Sold, nn));
+        DCALL(dalloc(d, &d->LX, nn));
+        DCALL(dalloc(d, &d->LXi, nn));
+        DCALL(dalloc(d, &d->LXiT, nn));
+        DCALL(dalloc(d, &d->lwarmP, n));
+    }
+    HIP_OK(hipMemcpyAsync(d->Xs, d->X, sizeof(double) * nn, hipMemcpyDeviceToDevice, d->st));
+    d->lzComputedP = 0;
+    d->psdpReady = 1;
+    return check_launch();
+}
+
+hdsdp_retcode hdsdp_dev_psdp_rhs(hdsdp_dev *d, double mu, double *AX, double *rhs) {
+    const int m = d->m, n = d->n;
+    if (!d->psdpReady) return HDSDP_RETCODE_FAILED;
+    // the dual matrix of this iteration, both triangles (HConeGetDual: interface/hdsdp_conic_sdp.c:2427-2438)
+    HIP_OK(hipMemcpyAsync(d->Sold, d->S[0], sizeof(double) * (size_t) n * n, hipMemcpyDeviceToDevice, d->st));
+    hdk::mirror_lower<double>(d->st, n, d->Sold, n);
+    // A X (hdsdp_psdp.c:183-186)
+    hdk::traces(d->st, d->c, 1, m, d->X, nullptr, 0.0, d->vecA_d, nullptr, false);
+    hdk::r1_quadforms(d->st, d->c, d->X, false, 1, m, 0.0, d->vecA_d, nullptr, d->H1);
+    // X S X (denseMatTripMultiplypy, hdsdp_psdp.c:21-52), then A(X S X) - mu A X (hdsdp_psdp.c:198-206)
+    hdk::gemm<double>(d->st, false, false, false, n, n, n, 1.0, d->Sold, n, 0, d->X, n, 0, 0.0, d->T1, n, 0, 1);
+    hdk::gemm<double>(d->st, true, false, false, n, n, n, 1.0, d->X, n, 0, d->T1, n, 0, 0.0, d->XSX, n, 0, 1);
+    hdk::traces(d->st, d->c, 1, m, d->XSX, nullptr, 0.0, d->vecB_d, nullptr, false);
+    hdk::r1_quadforms(d->st, d->c, d->XSX, false, 1, m, 0.0, d->vecB_d, nullptr, d->H1);
+    hdk::axpy(d->st, m, -mu, d->vecA_d, d->vecB_d);
+    DCALL(check_launch());
+    DCALL(download(d, AX, d->vecA_d, m, d->hvecA));
+    return download(d, rhs, d->vecB_d, m, d->hvecB);
+}
+
+hdsdp_retcode hdsdp_dev_psdp_direction(hdsdp_dev *d, double mu, int first, double *maxStep) {
+    const int n = d->n;
+    const long nn = (long) n * n;
+    if (!d->psdpReady) return HDSDP_RETCODE_FAILED;
+    // X S X + Xs dS Xs with dS the dual step the ratio test left in DUALSTEP (hdsdp_psdp.c:213-218)
+    hdk::mirror_lower<double>(d->st, n, d->S[2], n);
+    hdk::gemm<double>(d->st, false, false, false, n, n, n, 1.0, d->S[2], n, 0, d->Xs, n, 0, 0.0, d->T1, n, 0, 1);
+    hdk::gemm<double>(d->st, true, false, false, n, n, n, 1.0, d->Xs, n, 0, d->T1, n, 0, 1.0, d->XSX, n, 0, 1);
+    // dX = X - (X S X + Xs dS Xs) / mu (hdsdp_psdp.c:220-224)
+    HIP_OK(hipMemcpyAsync(d->dXp, d->X, sizeof(double) * nn, hipMemcpyDeviceToDevice, d->st));
+    hdk::axpy(d->st, nn, -1.0 / mu, d->XSX, d->dXp);
+    // the reference's products with dX read its lower triangle (fds_symv, linalg/dense_opts.c:46-51)
+    hdk::mirror_lower<double>(d->st, n, d->dXp, n);
+    if (first) {
+        int isPsd = 0;
+        DCALL(factor_x(d, &isPsd));
+        if (!isPsd) return HDSDP_RETCODE_FAILED;
+    }
+    // the primal Lanczos ratio test on L_X^-1 (-dX) L_X^-T (hdsdp_psdp.c:227-231, 58-66)
+    prof_begin(d, HDSDP_PROF_LANCZOS);
+    const LzOp op{d->LXi, d->LXiT, d->dXp, &d->lzComputedP, d->lwarmP};
+    DCALL(lanczos(d, op, maxStep));
+    return prof_end(d, HDSDP_PROF_LANCZOS);
+}
+
+hdsdp_retcode hdsdp_dev_psdp_step(hdsdp_dev *d, double alpha, int *isPsd, double *traceCX, double *XdotS) {
+    const int n = d->n;
+    if (!d->psdpReady) return HDSDP_RETCODE_FAILED;
+    // X += alpha dX, then the factor of the new X (hdsdp_psdp.c:237-267)
+    hdk::axpy(d->st, (long) n * n, alpha, d->dXp, d->X);
+    DCALL(factor_x(d, isPsd));
+    // <C, X> and <X, S> with the dual matrix of this iteration (hdsdp_psdp.c:268-271)
+    hdk::traces(d->st, d->c, 0, 1, d->X, nullptr, 0.0, d->vecA_d, nullptr, false);
+    hdk::r1_quadforms(d->st, d->c, d->X, true, 0, 1, 0.0, d->vecA_d, nullptr, d->H1);
+    hdk::xdots(d->st, n, d->X, d->Sold, d->sc_d + hdk::SC_AUX);
+    DCALL(check_launch());
+    DCALL(download(d, d->hvecB, d->vecA_d, 1, d->hvecA));
+    *traceCX = d->hvecB[0];
+    DCALL(read_scalars(d));
+    *XdotS = d->hsc[hdk::SC_AUX];
     return HDSDP_RETCODE_OK;
 }
 

@@ -136,7 +136,15 @@ hdsdp_retcode HDSDPCenter( hdsdp *s, double mu, double *y, int maxIter, double t
         HCALL(centre_eval(s, mu, y, g, work, &isPd));
         res = sqrt(dot(m, g, g)) / (1.0 + bnorm);
     }
-    if ( res > HMAX(tol, tolFloor) ) retcode = HDSDP_RETCODE_FAILED;
+    if ( res > HMAX(tol, tolFloor) ) {
+        retcode = HDSDP_RETCODE_FAILED;
+        goto exit_cleanup;
+    }
+    /* the Newton system at the centred point, solved but not stepped: its FP64 factor stays
+     * on the device with the Schur matrix of (mu, y), and HDSDPAdjoint at the same point
+     * solves with it instead of factoring again */
+    for ( int i = 0; i < m; ++i ) work[i] = g[i] / (mu * objScal * objScal);
+    HCALL(hdsdp_dev_kkt_solve_exact(s->sdp->dev, work, dy));
 exit_cleanup:
     if ( nIter ) *nIter = it;
     if ( relRes ) *relRes = res;

@@ -70,9 +70,8 @@ static void default_params( hdsdp *s ) {
     set_int_param(s, INT_PARAM_MAXITER, 500);
     set_int_param(s, INT_PARAM_CORRECTORA, 12);
     set_int_param(s, INT_PARAM_CORRECTORB, 12);
-    set_int_param(s, INT_PARAM_THREADS, 1);
-    /* Primal refinement is not carried by this build, so it is off by default. */
-    set_int_param(s, INT_PARAM_PSDP, 0);
+    set_int_param(s, INT_PARAM_THREADS, 12);
+    set_int_param(s, INT_PARAM_PSDP, 1);
     set_dbl_param(s, DBL_PARAM_ABSOPTTOL, 1e-08);
     set_dbl_param(s, DBL_PARAM_ABSFEASTOL, 1e-08);
     set_dbl_param(s, DBL_PARAM_RELOPTTOL, 1e-10);
@@ -406,8 +405,8 @@ hdsdp_retcode HDSDPOptimize( hdsdp *s, int dOptOnly ) {
     s->dTimeEnd = hdsdp_timestamp();
     hlog(s, "\nOptimization time: %3.1f seconds\n", s->dTimeEnd - s->dTimeBegin);
     if ( s->HStatus != HDSDP_INFEAS_OR_UNBOUNDED && s->HStatus != HDSDP_SUSPECT_INFEAS_OR_UNBOUNDED ) {
-        hdsdp_retcode rc2 = HDSDPCheckSolution(s, s->dErrs);
-        if ( retcode == HDSDP_RETCODE_OK ) retcode = rc2;
+        /* the solve's return code is replaced by the check's (interface/hdsdp.c:673-679) */
+        retcode = HDSDPCheckSolution(s, s->dErrs);
     }
     hlog(s, "DIMACS error metric:\n    %5.2e %5.2e %5.2e %5.2e %5.2e %5.2e \n",
          s->dErrs[0], s->dErrs[1], s->dErrs[2], s->dErrs[3], s->dErrs[4], s->dErrs[5]);
@@ -426,7 +425,7 @@ hdsdp_retcode HDSDPGetRowDual( hdsdp *s, double *pObjVal, double *dObjVal, doubl
 
 /* Recover X from the more accurate snapshot, else the less accurate one
  * (interface/hdsdp.c:711-745). */
-static hdsdp_retcode recover_primal( hdsdp *s, int *ok ) {
+hdsdp_retcode hdsdp_recover_primal( hdsdp *s, int *ok ) {
     double mu = s->dAccBarrierMaker;
     double *y = s->dAccRowDualMaker, *dy = s->dAccRowDualStepMaker;
     if ( mu <= 0.0 ) {
@@ -442,7 +441,7 @@ void HDSDPGetConeValues( hdsdp *s, int iCone, double *conePrimal, double *coneDu
     if ( iCone != 0 || !s->sdp ) return;
     int ok = 0;
     if ( conePrimal ) {
-        if ( recover_primal(s, &ok) == HDSDP_RETCODE_OK && ok ) {
+        if ( hdsdp_recover_primal(s, &ok) == HDSDP_RETCODE_OK && ok ) {
             hdsdp_dev_get_primal(s->sdp->dev, conePrimal);
         } else if ( s->verbose ) {
             printf("Recovery step is infeasible\n");
@@ -516,7 +515,7 @@ hdsdp_retcode HDSDPCheckSolution( hdsdp *s, double dErrs[6] ) {
     for ( int i = 0; i < m; ++i ) dualObj += s->rowRHS[i] * s->dRowDual[i];
 
     int ok = 0;
-    HCALL(recover_primal(s, &ok));
+    HCALL(hdsdp_recover_primal(s, &ok));
     if ( ok ) {
         HCALL(hdsdp_dev_primal_stats(s->sdp->dev, AX, &primalObj, &compl, &traceX));
         /* The reference takes the largest eigenvalue of X here (fds_syev with m = 1:

@@ -176,8 +176,8 @@ hdsdp_retcode sdp_cone_schur( sdp_cone *cone, host_kkt *kkt, int typeKKT ) {
     double *asinv = NULL, *asinvrd = NULL;
     double traceSinv = 0.0;
     int m = cone->nRow;
-    if ( typeKKT == KKT_TYPE_PRIMAL || typeKKT == KKT_TYPE_HOMOGENEOUS ) {
-        /* Primal refinement and the self-dual embedding are not carried by this build. */
+    if ( typeKKT == KKT_TYPE_HOMOGENEOUS ) {
+        /* The self-dual embedding is not carried by this build. */
         retcode = HDSDP_RETCODE_FAILED;
         goto exit_cleanup;
     }

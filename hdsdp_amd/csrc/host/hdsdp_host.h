@@ -283,5 +283,9 @@ hdsdp_retcode kkt_regularize( hdsdp *s, double reg );
 /* ipm.c */
 hdsdp_retcode hdsdp_solve( hdsdp *s, int dOnly );
 void hdsdp_record_log( hdsdp *s, int phase );
+/* X of the dual method's last accurate (else inaccurate) barrier point, left in the device's
+ * primal buffer; *ok = 0 when the recovery step is infeasible (HDSDPGetConeValues,
+ * interface/hdsdp.c:711-745) */
+hdsdp_retcode hdsdp_recover_primal( hdsdp *s, int *ok );
 
 #endif /* HDSDP_HOST_H */

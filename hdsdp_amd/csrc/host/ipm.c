@@ -19,6 +19,7 @@
 #define ALGO_DUAL_INFEAS    (0)
 #define ALGO_DUAL_HSD       (1)
 #define ALGO_DUAL_POTENTIAL (2)
+#define ALGO_PRIMAL         (3)
 
 double hdsdp_timestamp( void ) {
     struct timeval t;
@@ -79,7 +80,7 @@ void hdsdp_record_log( hdsdp *s, int phase ) {
     r[1] = s->nIterCount + 1;
     r[2] = s->pObjVal;
     r[3] = s->dObjVal;
-    r[4] = ( phase == ALGO_DUAL_POTENTIAL ) ? s->pInfeas : s->dInfeas;
+    r[4] = ( phase == ALGO_DUAL_POTENTIAL || phase == ALGO_PRIMAL ) ? s->pInfeas : s->dInfeas;
     r[5] = s->dBarrierMu;
     r[6] = s->dDStep;
     r[7] = s->dProxNorm;
@@ -688,6 +689,81 @@ static int primal_infeas_check( hdsdp *s, int forceDetect ) {
     return isRay;
 }
 
+/* The primal method, HPSDPInit + HPSDPOptimize (interface/hdsdp_psdp.c:78-295), entered
+ * from the dual method once its gap is below 10% (interface/hdsdp_algo.c:1803-1810): from
+ * the recovered primal X, at most five primal-dual steps on the Schur matrix of W = X
+ * (built and factored once), each solving M dy = A(X S X) - mu A X, the dual step length
+ * by the dual ratio test on dS = -sum dy_i A_i, the primal direction
+ * dX = X - (X S X + Xs dS Xs) / mu with Xs the starting X, the primal step by the primal
+ * Lanczos solver, both steps at 95% capped at 1; mu = min(mu, <X, S> / (5 n^2)).  The
+ * loop test reads the dual method's last complementarity (interface/hdsdp_psdp.c:
+ * 287-289), which these steps do not update. */
+static hdsdp_retcode psdp_optimize( hdsdp *s ) {
+    hdsdp_retcode retcode = HDSDP_RETCODE_OK;
+    const int m = s->nRows, n = s->sdp->nCol;
+    hdsdp_dev *dev = s->sdp->dev;
+    const double relOpt = get_dbl_param(s, DBL_PARAM_RELOPTTOL), absOpt = get_dbl_param(s, DBL_PARAM_ABSOPTTOL);
+    const double pdScal = get_dbl_feature(s, DBL_FEATURE_OBJSCALING) * get_dbl_feature(s, DBL_FEATURE_RHSSCALING);
+    double *AX = NULL, *rhs = NULL, mu = s->dBarrierMu;
+    int ok = 0;
+    /* one SDP cone and no dual infeasibility (interface/hdsdp_psdp.c:85-90) */
+    if ( s->dInfeas > 0.0 ) {
+        retcode = HDSDP_RETCODE_FAILED;
+        goto exit_cleanup;
+    }
+    HALLOC(AX, double, m);
+    HALLOC(rhs, double, m);
+    /* X from the dual method's recovery (HDSDPGetConeValues, interface/hdsdp_psdp.c:131) */
+    HCALL(hdsdp_recover_primal(s, &ok));
+    if ( !ok ) {
+        hlog(s, "Recovery step is infeasible\n");
+        retcode = HDSDP_RETCODE_FAILED;
+        goto exit_cleanup;
+    }
+    HCALL(hdsdp_dev_psdp_init(dev));
+    hlog(s, "Primal solver is initialized. \n");
+    HCALL(kkt_build(s, KKT_TYPE_PRIMAL));
+    hlog(s, "KKT Build up: %f s", hdsdp_timestamp() - s->dTimeBegin);
+    HCALL(kkt_factorize(s));
+    hlog(s, "KKT Factorize: %f s", hdsdp_timestamp() - s->dTimeBegin);
+    for ( int it = 0; it < 5; ++it ) {
+        double pInf = 0.0, maxD = 0.0, maxP = 0.0, pObj = 0.0, XdotS = 0.0;
+        int isPsd = 0, isInterior = 0;
+        HCALL(hdsdp_dev_psdp_rhs(dev, mu, AX, rhs));
+        for ( int i = 0; i < m; ++i ) pInf += (s->rowRHS[i] - AX[i]) * (s->rowRHS[i] - AX[i]);
+        pInf = sqrt(pInf);
+        HCALL(kkt_solve(s, rhs, s->dRowDualStep));
+        HCALL(sdp_cone_ratio_test(s->sdp, 0.0, s->dRowDualStep, 1.0, BUFFER_DUALVAR, &maxD));
+        HCALL(hdsdp_dev_psdp_direction(dev, mu, it == 0, &maxP));
+        maxP = HMIN(0.95 * maxP, 1.0);
+        maxD = HMIN(0.95 * maxD, 1.0);
+        HCALL(hdsdp_dev_psdp_step(dev, maxP, &isPsd, &pObj, &XdotS));
+        for ( int i = 0; i < m; ++i ) s->dRowDual[i] += maxD * s->dRowDualStep[i];
+        HCALL(sdp_cone_update(s->sdp, 1.0, s->dRowDual));
+        HCALL(sdp_cone_interior(s->sdp, 1.0, s->dRowDual, &isInterior));
+        if ( !isInterior || !isPsd ) {
+            retcode = HDSDP_RETCODE_FAILED;
+            goto exit_cleanup;
+        }
+        double dObj = vdot(m, s->rowRHS, s->dRowDual);
+        mu = HMIN(mu, (XdotS / n) / (5.0 * n));
+        s->pObjVal = pObj;
+        s->dObjVal = dObj;
+        s->pInfeas = pInf;
+        s->dBarrierMu = mu;
+        s->dDStep = maxD;
+        s->nIterCount += 1;
+        hlog(s, "    %5d  %+15.8e  %+15.8e  %8.2e  %8.2e  %5.2f %5.2f     %4.1f \n", s->nIterCount + 1, s->pObjVal,
+             s->dObjVal, s->pInfeas, s->dBarrierMu, maxP, s->dDStep, hdsdp_timestamp() - s->dTimeBegin);
+        hdsdp_record_log(s, ALGO_PRIMAL);
+        if ( s->comp < (fabs(s->pObjVal) + fabs(s->dObjVal) + 1.0) * relOpt && s->comp < absOpt / pdScal ) break;
+    }
+exit_cleanup:
+    free(AX);
+    free(rhs);
+    return retcode;
+}
+
 /* interface/hdsdp_algo.c:1657-1846 */
 static hdsdp_retcode phase_b( hdsdp *s ) {
     hdsdp_retcode retcode = HDSDP_RETCODE_OK;
@@ -769,9 +845,8 @@ static hdsdp_retcode phase_b( hdsdp *s ) {
             break;
         }
         if ( s->comp < (fabs(s->pObjVal) + fabs(s->dObjVal) + 1.0) * 0.1 && iPrimalMethod ) {
-            hlog(s, "Primal refinement (PSDP) is not carried by this build; set INT_PARAM_PSDP to 0.\n");
-            retcode = HDSDP_RETCODE_FAILED;
-            goto exit_cleanup;
+            retcode = psdp_optimize(s);
+            break;
         }
         if ( s->nSmallStep > 3 ) {
             s->HStatus = HDSDP_NUMERICAL;

@@ -97,15 +97,16 @@ hdsdp_retcode hdsdp_dev_ratio_test( hdsdp_dev *dev, int whichBuffer, double *max
 
 /* ---- Schur complement (coneBuildSchur slot) ----------------------------- */
 
-/* Build with W = S^-1 from the dual factor:
+/* Build with W = S^-1 from the dual factor (typeKKT INFEASIBLE, CORRECTOR) or
+ * W = X, the primal method's iterate (PRIMAL; every constraint SPARSE or SPR1,
+ * as the reference's strategy M5 requires, after hdsdp_dev_psdp_init):
  *   M_ij = <A_i, W A_j W> (lower triangle, not for CORRECTOR),
  *   dASinv[i] = <A_i, W>, dASinvRdSinv[i] = Rd <A_i, W W>,
  *   *dTraceSinv = tr(W) when Rd != 0 (not for CORRECTOR), else 0.
- * typeKKT INFEASIBLE (0) and CORRECTOR (1) only: HOMOGENEOUS (2, the HSD
- * scalars) and PRIMAL (3, W = X) return HDSDP_RETCODE_FAILED; the host runs
- * without the HSD phase and with INT_PARAM_PSDP = 0.
+ * HOMOGENEOUS (2, the HSD scalars) returns HDSDP_RETCODE_FAILED: the host runs
+ * without the HSD phase.
  * Replaces sdpDenseConeGetKKT and its strategies M2-M5
- * (interface/hdsdp_conic_sdp.c:683-1052, 1722-1804). */
+ * (interface/hdsdp_conic_sdp.c:683-1052, 1722-1872). */
 hdsdp_retcode hdsdp_dev_schur( hdsdp_dev *dev, int typeKKT, double dualResidual,
                                double *dASinv, double *dASinvRdSinv, double *dTraceSinv );
 
@@ -160,6 +161,25 @@ hdsdp_retcode hdsdp_dev_primal_stats( hdsdp_dev *dev, double *AX, double *traceC
 /* Copy the last X (n x n, full) or S (n x n, symmetrized) to the host. */
 hdsdp_retcode hdsdp_dev_get_primal( hdsdp_dev *dev, double *X );
 hdsdp_retcode hdsdp_dev_get_dual( hdsdp_dev *dev, double *S );
+
+/* ---- primal method (interface/hdsdp_psdp.c) ------------------------------ */
+
+/* Start the primal iterate at the X of the last hdsdp_dev_primal and keep a
+ * copy of it as the fixed scaling Xs; the primal Lanczos solver starts afresh.
+ * Replaces the device half of HPSDPInit (interface/hdsdp_psdp.c:78-140). */
+hdsdp_retcode hdsdp_dev_psdp_init( hdsdp_dev *dev );
+/* AX[i] = <A_i, X> and rhs[i] = <A_i, X S X> - mu AX[i] with S the current
+ * dual matrix (DUALVAR), kept for the step (interface/hdsdp_psdp.c:180-206). */
+hdsdp_retcode hdsdp_dev_psdp_rhs( hdsdp_dev *dev, double mu, double *AX, double *rhs );
+/* After the dual ratio test left dS in DUALSTEP: dX = X - (X S X + Xs dS Xs) / mu;
+ * on the first call X is factored (FAILED if not positive definite); then the
+ * largest step a with X + a dX psd, by the primal Lanczos solver on
+ * L_X^-1 (-dX) L_X^-T (interface/hdsdp_psdp.c:208-231, 58-66). */
+hdsdp_retcode hdsdp_dev_psdp_direction( hdsdp_dev *dev, double mu, int first, double *maxStep );
+/* X += alpha dX and X factored again (*isPsd); <C, X> and <X, S> with the dual
+ * matrix of hdsdp_dev_psdp_rhs (interface/hdsdp_psdp.c:237-271). */
+hdsdp_retcode hdsdp_dev_psdp_step( hdsdp_dev *dev, double alpha, int *isPsd, double *traceCX,
+                                   double *XdotS );
 
 /* ---- adjoint (no counterpart in the reference) -------------------------- */
 

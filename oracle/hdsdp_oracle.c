@@ -55,6 +55,12 @@ struct hdsdp_dev {
     int useJacobi, paramUseJacobi, maxIter, nRestartFreq;
     double absTol, relTol;
     olanczos lz;
+    /* primal method (interface/hdsdp_psdp.c): the iterate is X; Xs its starting value,
+     * dXp the direction, XSX the buffer X S X (+ Xs dS Xs), Sold the dual matrix of the
+     * iteration, LX the Cholesky factor of X; lzP the primal method's own Lanczos solver */
+    double *Xs, *dXp, *XSX, *Sold, *LX;
+    olanczos lzP;
+    int psdpReady;
     double stats[6];
 };
 
@@ -265,6 +271,27 @@ static void osandwich( const ocoeff *A, int n, const double *W, double *tmpA, do
 
 /* ------------------------------------------------------------- lifecycle */
 
+static void lz_alloc( olanczos *lz, int n ) {
+    lz->n = n;
+    lz->maxDim = 30;
+    lz->nComputed = 0;
+    lz->v = (double *) calloc(n, sizeof(double));
+    lz->w = (double *) calloc(n, sizeof(double));
+    lz->z1 = (double *) calloc(n, sizeof(double));
+    lz->z2 = (double *) calloc(n, sizeof(double));
+    lz->warm = (double *) calloc(n, sizeof(double));
+    lz->V = (double *) calloc((size_t) n * (lz->maxDim + 1), sizeof(double));
+    lz->H = (double *) calloc((size_t) (lz->maxDim + 1) * (lz->maxDim + 1), sizeof(double));
+    lz->Y = (double *) calloc((size_t) lz->maxDim * 2, sizeof(double));
+    lz->U = (double *) calloc((size_t) lz->maxDim * lz->maxDim, sizeof(double));
+    lz->d = (double *) calloc(lz->maxDim, sizeof(double));
+}
+
+static void lz_free( olanczos *lz ) {
+    free(lz->v); free(lz->w); free(lz->z1); free(lz->z2); free(lz->warm);
+    free(lz->V); free(lz->H); free(lz->Y); free(lz->U); free(lz->d);
+}
+
 hdsdp_retcode hdsdp_dev_create( hdsdp_dev **pdev, int nRow, int nCol, int precision ) {
     hdsdp_dev *d = (hdsdp_dev *) calloc(1, sizeof(hdsdp_dev));
     if ( !d ) return HDSDP_RETCODE_MEMORY;
@@ -298,18 +325,7 @@ hdsdp_retcode hdsdp_dev_create( hdsdp_dev **pdev, int nRow, int nCol, int precis
     d->maxIter = OMAX(50, nRow / 20);
     d->nRestartFreq = 20;
     olanczos *lz = &d->lz;
-    lz->n = nCol;
-    lz->maxDim = 30;
-    lz->v = (double *) calloc(nCol, sizeof(double));
-    lz->w = (double *) calloc(nCol, sizeof(double));
-    lz->z1 = (double *) calloc(nCol, sizeof(double));
-    lz->z2 = (double *) calloc(nCol, sizeof(double));
-    lz->warm = (double *) calloc(nCol, sizeof(double));
-    lz->V = (double *) calloc((size_t) nCol * (lz->maxDim + 1), sizeof(double));
-    lz->H = (double *) calloc((size_t) (lz->maxDim + 1) * (lz->maxDim + 1), sizeof(double));
-    lz->Y = (double *) calloc((size_t) lz->maxDim * 2, sizeof(double));
-    lz->U = (double *) calloc((size_t) lz->maxDim * lz->maxDim, sizeof(double));
-    lz->d = (double *) calloc(lz->maxDim, sizeof(double));
+    lz_alloc(lz, nCol);
     if ( !d->M || !d->Mchol || !d->W || !lz->V ) return HDSDP_RETCODE_MEMORY;
     *pdev = d;
     return HDSDP_RETCODE_OK;
@@ -365,9 +381,9 @@ void hdsdp_dev_destroy( hdsdp_dev **pdev ) {
     free(d->W); free(d->W2); free(d->X); free(d->tmp); free(d->tmp2); free(d->vecn);
     free(d->M); free(d->Mchol); free(d->jacobi);
     free(d->r); free(d->rnew); free(d->p); free(d->z); free(d->Mp); free(d->x); free(d->rhsbuf);
-    olanczos *lz = &d->lz;
-    free(lz->v); free(lz->w); free(lz->z1); free(lz->z2); free(lz->warm);
-    free(lz->V); free(lz->H); free(lz->Y); free(lz->U); free(lz->d);
+    lz_free(&d->lz);
+    lz_free(&d->lzP);
+    free(d->Xs); free(d->dXp); free(d->XSX); free(d->Sold); free(d->LX);
     free(d);
     *pdev = NULL;
 }
@@ -450,16 +466,20 @@ static double lz_normalize( int n, double *a ) {
     return nrm;
 }
 
-/* x -> L^-1 (-dS) L^-T x (interface/hdsdp_conic_sdp.c:462-505) */
-static void lz_mvec( hdsdp_dev *d, const double *L, const double *in, double *out ) {
+/* x -> L^-1 (-D) L^-T x: the dual cone's operator with the factor of a dual buffer and
+ * D = dS (interface/hdsdp_conic_sdp.c:462-505), the primal method's with the factor of X
+ * and D = the primal direction (interface/hdsdp_psdp.c:58-66); D's lower triangle. */
+static void lz_mvec( hdsdp_dev *d, const double *L, const double *D, const double *in, double *out ) {
     int n = d->n;
     memcpy(out, in, sizeof(double) * n);
     otrsv(n, L, out, 1);
-    osymv(n, -1.0, d->S[2], out, d->vecn);
+    osymv(n, -1.0, D, out, d->vecn);
     memcpy(out, d->vecn, sizeof(double) * n);
     otrsv(n, L, out, 0);
     d->stats[3] += 1;
 }
+
+static hdsdp_retcode lanczos_run( hdsdp_dev *d, olanczos *lz, const double *L, const double *D, double *maxStep );
 
 hdsdp_retcode hdsdp_dev_ratio_test( hdsdp_dev *d, int which, double *maxStep ) {
     int n = d->n, f = ( which == 0 ) ? 0 : 1;
@@ -468,8 +488,12 @@ hdsdp_retcode hdsdp_dev_ratio_test( hdsdp_dev *d, int which, double *maxStep ) {
         *maxStep = ( ds > 0.0 ) ? HDSDP_INFINITY : (-s / ds);
         return HDSDP_RETCODE_OK;
     }
-    olanczos *lz = &d->lz;
-    const double *L = d->L[f];
+    return lanczos_run(d, &d->lz, d->L[f], d->S[2], maxStep);
+}
+
+/* HLanczosSolve (linalg/hdsdp_lanczos.c:161-292) */
+static hdsdp_retcode lanczos_run( hdsdp_dev *d, olanczos *lz, const double *L, const double *D, double *maxStep ) {
+    int n = d->n;
     int nV = n, nH = lz->maxDim + 1;
 #define HH(i, j) lz->H[(size_t) nH * (j) + (i)]
 #define VV(i, j) lz->V[(size_t) nV * (j) + (i)]
@@ -494,7 +518,7 @@ hdsdp_retcode hdsdp_dev_ratio_test( hdsdp_dev *d, int which, double *maxStep ) {
     if ( trace < 0 ) trace = getenv("HDSDP_ORACLE_LZ_TRACE") != NULL;
     if ( trace ) printf("Lanczos call %d\n", lz->nComputed);
     for ( int k = 0; k < lz->maxDim; ++k ) {
-        lz_mvec(d, L, lz->v, lz->w);
+        lz_mvec(d, L, D, lz->v, lz->w);
         if ( k > 0 ) {
             double neg = -HH(k, k - 1);
             for ( int i = 0; i < n; ++i ) lz->w[i] += neg * VV(i, k - 1);
@@ -527,7 +551,7 @@ hdsdp_retcode hdsdp_dev_ratio_test( hdsdp_dev *d, int which, double *maxStep ) {
                     for ( int c = 0; c < kp1; ++c ) s += VV(i, c) * lz->Y[kp1 + c];
                     lz->z1[i] = s;
                 }
-                lz_mvec(d, L, lz->z1, lz->z2);
+                lz_mvec(d, L, D, lz->z1, lz->z2);
                 memcpy(lz->warm, lz->z2, sizeof(double) * n);
                 for ( int i = 0; i < n; ++i ) lz->z2[i] += -e1 * lz->z1[i];
                 double r1 = onrm2(n, lz->z2);
@@ -536,7 +560,7 @@ hdsdp_retcode hdsdp_dev_ratio_test( hdsdp_dev *d, int which, double *maxStep ) {
                     for ( int c = 0; c < kp1; ++c ) s += VV(i, c) * lz->Y[c];
                     lz->z2[i] = s;
                 }
-                lz_mvec(d, L, lz->z2, lz->z1);
+                lz_mvec(d, L, D, lz->z2, lz->z1);
                 for ( int i = 0; i < n; ++i ) lz->z1[i] += -e1 * lz->z2[i];
                 double r2 = onrm2(n, lz->z1);
                 double diff = e1 - e2 - r2;
@@ -587,8 +611,17 @@ hdsdp_retcode hdsdp_dev_schur( hdsdp_dev *d, int typeKKT, double Rd, double *asi
                                double *traceSinv ) {
     int n = d->n, m = d->m;
     size_t nn = (size_t) n * n;
-    if ( typeKKT == 2 || typeKKT == 3 ) return HDSDP_RETCODE_FAILED;
-    oinverse(d, d->L[0], d->W);
+    if ( typeKKT == 2 ) return HDSDP_RETCODE_FAILED;
+    if ( typeKKT == 3 ) {
+        /* PRIMAL: strategy M5 with W := X (interface/hdsdp_conic_sdp.c:1741-1745, 1813-1820);
+         * no DENSE or DSR1 constraint (linalg/hdsdp_sdpdata.c:1961-1965, 1995-1999) */
+        if ( !d->psdpReady ) return HDSDP_RETCODE_FAILED;
+        for ( int i = 1; i <= m; ++i )
+            if ( !is_sparse_kind(d->A[i].type) && d->A[i].type != HDSDP_COEFF_ZERO ) return HDSDP_RETCODE_FAILED;
+        memcpy(d->W, d->X, sizeof(double) * nn);
+    } else {
+        oinverse(d, d->L[0], d->W);
+    }
     ogemm(n, d->W, d->W, d->W2);
     *traceSinv = 0.0;
     for ( int i = 0; i < m; ++i ) {
@@ -902,6 +935,79 @@ hdsdp_retcode hdsdp_dev_get_primal( hdsdp_dev *d, double *X ) {
 hdsdp_retcode hdsdp_dev_get_dual( hdsdp_dev *d, double *S ) {
     memcpy(S, d->S[0], sizeof(double) * (size_t) d->n * d->n);
     osymmetrize_lower(d->n, S);
+    return HDSDP_RETCODE_OK;
+}
+
+/* ------------------------------------------------------- primal method */
+
+/* X S X with X full and S's lower triangle (denseMatTripMultiplypy,
+ * interface/hdsdp_psdp.c:21-52): SX = S X column by column, then XSX(i, j) += X_i . SX_j. */
+static void otrip( int n, const double *S, const double *X, double *SX, double *XSX ) {
+    for ( int j = 0; j < n; ++j ) osymv(n, 1.0, S, X + (size_t) j * n, SX + (size_t) j * n);
+    for ( int j = 0; j < n; ++j ) {
+        const double *sxj = SX + (size_t) j * n;
+        for ( int i = 0; i < j; ++i ) {
+            double v = odot(n, sxj, X + (size_t) i * n);
+            FE(XSX, n, j, i) += v;
+            FE(XSX, n, i, j) += v;
+        }
+        FE(XSX, n, j, j) += odot(n, sxj, X + (size_t) j * n);
+    }
+}
+
+hdsdp_retcode hdsdp_dev_psdp_init( hdsdp_dev *d ) {
+    size_t nn = (size_t) d->n * d->n;
+    if ( !d->Xs ) {
+        d->Xs = (double *) calloc(nn, sizeof(double));
+        d->dXp = (double *) calloc(nn, sizeof(double));
+        d->XSX = (double *) calloc(nn, sizeof(double));
+        d->Sold = (double *) calloc(nn, sizeof(double));
+        d->LX = (double *) calloc(nn, sizeof(double));
+        lz_alloc(&d->lzP, d->n);
+        if ( !d->Xs || !d->dXp || !d->XSX || !d->Sold || !d->LX || !d->lzP.V ) return HDSDP_RETCODE_MEMORY;
+    }
+    memcpy(d->Xs, d->X, sizeof(double) * nn);
+    d->lzP.nComputed = 0;
+    d->psdpReady = 1;
+    return HDSDP_RETCODE_OK;
+}
+
+hdsdp_retcode hdsdp_dev_psdp_rhs( hdsdp_dev *d, double mu, double *AX, double *rhs ) {
+    int n = d->n, m = d->m;
+    size_t nn = (size_t) n * n;
+    if ( !d->psdpReady ) return HDSDP_RETCODE_FAILED;
+    memcpy(d->Sold, d->S[0], sizeof(double) * nn);
+    osymmetrize_lower(n, d->Sold);
+    for ( int i = 0; i < m; ++i ) AX[i] = otrace_AB(&d->A[i + 1], n, d->X, d->vecn);
+    memset(d->XSX, 0, sizeof(double) * nn);
+    otrip(n, d->Sold, d->X, d->tmp, d->XSX);
+    for ( int i = 0; i < m; ++i ) rhs[i] = otrace_AB(&d->A[i + 1], n, d->XSX, d->vecn) - mu * AX[i];
+    return HDSDP_RETCODE_OK;
+}
+
+hdsdp_retcode hdsdp_dev_psdp_direction( hdsdp_dev *d, double mu, int first, double *maxStep ) {
+    int n = d->n;
+    size_t nn = (size_t) n * n;
+    if ( !d->psdpReady ) return HDSDP_RETCODE_FAILED;
+    osymmetrize_lower(n, d->S[2]);
+    otrip(n, d->S[2], d->Xs, d->tmp, d->XSX);
+    for ( size_t e = 0; e < nn; ++e ) d->dXp[e] = d->X[e] - d->XSX[e] / mu;
+    if ( first ) {
+        memcpy(d->LX, d->X, sizeof(double) * nn);
+        if ( ochol(n, d->LX) != 0 ) return HDSDP_RETCODE_FAILED;
+    }
+    return lanczos_run(d, &d->lzP, d->LX, d->dXp, maxStep);
+}
+
+hdsdp_retcode hdsdp_dev_psdp_step( hdsdp_dev *d, double alpha, int *isPsd, double *traceCX, double *XdotS ) {
+    int n = d->n;
+    size_t nn = (size_t) n * n;
+    if ( !d->psdpReady ) return HDSDP_RETCODE_FAILED;
+    for ( size_t e = 0; e < nn; ++e ) d->X[e] += alpha * d->dXp[e];
+    memcpy(d->LX, d->X, sizeof(double) * nn);
+    *isPsd = ( ochol(n, d->LX) == 0 );
+    *traceCX = otrace_AB(&d->A[0], n, d->X, d->vecn);
+    *XdotS = odot(nn, d->X, d->Sold);
     return HDSDP_RETCODE_OK;
 }
 

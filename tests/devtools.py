@@ -294,8 +294,9 @@ def solve_like_golden(lib, name, **kw):
 
     inst, fix = load_golden(name)
     opts = dict(fix["options"])
+    opts.setdefault("psdp", 0)
     opts.update(kw)
-    res = run_solver(lib, inst, psdp=0, **opts)
+    res = run_solver(lib, inst, **opts)
     return inst, fix, res
 
 

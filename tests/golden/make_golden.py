@@ -42,6 +42,12 @@ RUNNER_DENSE_FMA = os.path.join(ROOT, "oracle", "_ref", "dense_fma", "hdsdp_ref_
 #             PARDISO (interface/hdsdp_conic_sdp.c:65-97); the dense-dual build
 #             (oracle/_ref/dense) is run as well and its result kept beside the default one
 #   "heavy":  the spread runs use the default thread count only (one thread takes minutes)
+#   "dense_primary": the primary result comes from the dense-dual build (the default build
+#             stops with a segmentation fault on this instance: its primal method symmetrises
+#             the dual step as a dense n x n array while PARDISO holds it sparse,
+#             interface/hdsdp_psdp.c:218), and so do the spread runs
+# Options may carry "psdp": 1, the reference's default primal method (interface/hdsdp.c:384);
+# every other case runs with it off.
 CASES = {
     "c1_gap1e-6": (lambda: I.maxcut(200, 0.05, 1, "pm1"), {"relopt": 1e-6, "absopt": 1e30}, {"sparse"}),
     "c1_default": (lambda: I.maxcut(200, 0.05, 1, "pm1"), {}, {"sparse"}),
@@ -58,13 +64,22 @@ CASES = {
     "dense_n200_m400": (lambda: I.dense_random(200, 400, 34), {"relopt": 1e-6, "absopt": 1e30}, {"heavy"}),
     "rankone_n200_m2000": (lambda: I.rank_one(200, 2000, 20, 4, form="csc"), {"relopt": 1e-6, "absopt": 1e30},
                            {"heavy"}),
+    # the primal method on (it starts once the dual gap is below 10% and ends the solve)
+    "theta_n50_psdp": (lambda: I.lovasz_theta(50, 0.3, 21), {"psdp": 1}, set()),
+    "maxcut_n60_psdp": (lambda: I.maxcut(60, 0.2, 22, "unif"), {"psdp": 1}, {"dense_primary"}),
+    "c1_psdp": (lambda: I.maxcut(200, 0.05, 1, "pm1"), {"psdp": 1}, {"dense_primary"}),
+    "c2_psdp": (lambda: I.lovasz_theta(400, 0.1, 2), {"psdp": 1}, {"heavy"}),
+    "c5_psdp": (lambda: I.maxcut(2000, 0.01, 5, "unif"), {"psdp": 1}, {"dense_primary", "heavy"}),
 }
 
 # MKL's instruction-set code paths (MKL_CBWR): the same reference build on the same input,
 # as it runs on CPUs of different generations -- each path sums in its own order.  Heavy
 # cases run the first three.
-CBWR_PATHS = ("", "COMPATIBLE", "AVX2", "AVX", "SSE4_2")
+CBWR_PATHS = ("", "COMPATIBLE", "AVX2", "AVX", "SSE4_2", "SSE4_1", "SSSE3", "SSE3", "SSE2", "AVX512")
 
+# the reference prints its first primal-method log line after two timing lines without a
+# line break (interface/hdsdp_psdp.c:175-178)
+PSDP_PREFIX = re.compile(r"^KKT Build up: \S+ sKKT Factorize: \S+ s")
 LOG_RE = re.compile(r"^\s+(\d+)\s+([+-]\d\.\d+e[+-]\d+)\s+([+-]\d\.\d+e[+-]\d+)\s+(\S+)\s+(\S+)\s+(\S+)\s+(\S+)\s+(\S+)")
 
 
@@ -96,6 +111,9 @@ def run_reference(inst, opts, timeout=3000, runner=RUNNER, cbwr=""):
             phase = 0
         elif "feasible dual method" in line:
             phase = 2
+        elif "Primal solver is initialized" in line:
+            phase = 3
+        line = PSDP_PREFIX.sub("", line)
         mt = LOG_RE.match(line)
         if mt and phase >= 0:
             log.append([phase, int(mt.group(1)), float(mt.group(2)), float(mt.group(3)), float(mt.group(5)),
@@ -111,7 +129,9 @@ def spread_runs(inst, opts, flags, primary):
     over all of them (primary included), relative to the primary result, and the runs'
     status and iteration counts."""
     builds = [("default", RUNNER), ("fma", RUNNER_FMA)]
-    if "sparse" in flags:
+    if "dense_primary" in flags:
+        builds = [("dense", RUNNER_DENSE), ("dense_fma", RUNNER_DENSE_FMA)]
+    elif "sparse" in flags:
         builds += [("dense", RUNNER_DENSE), ("dense_fma", RUNNER_DENSE_FMA)]
     threads = [None] if "heavy" in flags else [None, 1]
     paths = CBWR_PATHS[:3] if "heavy" in flags else CBWR_PATHS
@@ -145,11 +165,12 @@ def main(names):
     for name in names or CASES:
         make, opts, flags = CASES[name]
         inst = make()
-        res, y, log, _ = run_reference(inst, opts)
+        res, y, log, _ = run_reference(inst, opts, runner=RUNNER_DENSE if "dense_primary" in flags else RUNNER)
         fix = {"name": name, "instance": inst.name, "digest": instance_digest(inst), "options": opts,
                "status": res["status"], "iters": res["iters"], "pobj": res["pobj"], "dobj": res["dobj"],
                "mu": res["mu"], "dimacs": res["dimacs"], "ref_seconds": res["time"],
-               "ref_threads": res.get("threads"), "log": log}
+               "ref_threads": res.get("threads"), "ref_build": "dense" if "dense_primary" in flags else "default",
+               "log": log}
         print(name, res["status"], res["iters"], res["dobj"], f"{res['time']:.2f}s", flush=True)
         if "sparse" in flags:
             dres, dy, dlog, _ = run_reference(inst, opts, runner=RUNNER_DENSE)

@@ -185,14 +185,17 @@ def test_schur_mixed_precision_gram(gpu_lib, name, monkeypatch):
     assert rel(a, [np.sum(Ai * W) for Ai in A]) < 1e-10
 
 
+@pytest.mark.parametrize("planes", ["1", "0"])
 @pytest.mark.parametrize("n,m", [(260, 300), (130, 700), (37, 90)])
-def test_dense_schur_mixed_planes(gpu_lib, n, m):
+def test_dense_schur_mixed_planes(gpu_lib, n, m, planes, monkeypatch):
     """The mixed-precision dense Schur matrix from bf16 planes (L^-T and A_p split by the
     producing kernels, V_p = A_p L^-T written as planes by the product's epilogue, the
     packed T_p padded to a multiple of 32 with zeros): several 128-wide tiles with ragged
     edges (n = 260), the split-K Gram in parts plus a ragged last part (m = 700), and n not
     a multiple of 8 (n = 37, padded planes).  Within 1e-6 relative of the FP64 closed form
-    and within 1e-6 of the FP64 device build."""
+    and within 1e-6 of the FP64 device build.  Both mixed kernels: the planes (split once by
+    the producers) and the products that split their FP32 operands in the loop."""
+    monkeypatch.setenv("HDSDP_BF16_PLANES", planes)
     rng = np.random.default_rng(19)
     coeffs = dt.rand_coeffs(rng, n, m, [dt.DENSE])
     y, eye = dt.interior_point(coeffs, rng)
