@@ -71,7 +71,8 @@ def build_oracle() -> None:
 def build_reference() -> bool:
     if not os.path.isdir("/root/reference"):
         return False
-    _run(["make", "-s", "-j8", "-C", ROOT, "-f", "oracle/Makefile.ref"])
+    # the reference builds, then the boundary test shim over the product and oracle libraries
+    _run(["make", "-s", "-j8", "-C", ROOT, "-f", "oracle/Makefile.ref", "all", "glue"])
     return True
 
 

@@ -1,0 +1,67 @@
+"""The reference's own host solver over the device ABI (the drop-in boundary).
+
+tests/glue/hdsdp_gpu_glue.c is linked with the reference's unmodified object files
+(oracle/Makefile.ref, target `glue`): its interface/hdsdp_algo.c, hdsdp.c and
+hdsdp_schur.c drive the iterations, while the DENSE_SDP cone slots
+(interface/hdsdp_conic.c:117-146) and the Schur solver's hdsdp_linsys_fp table
+(linalg/def_hdsdp_linsolver.h:34-57) forward to include/hdsdp_gpu.h.  Two builds:
+
+  hdsdp_glue_run         over hdsdp_amd/lib/libhdsdp_amd.so (the product, GPU)
+  hdsdp_glue_run_oracle  over oracle/_build/libhdsdp_oracle.so (the CPU oracle's
+                         implementation of the same ABI: checks the shim without a GPU)
+
+A run must match the reference's stored result for the instance to the parity bar
+(status, iteration count, y*, objectives within the reference's own spread) and must
+have sent its work through the slots (the shim's call counts)."""
+import json
+import os
+import types
+
+import pytest
+
+import devtools as dt
+import golden.make_golden as mg
+
+GLUE_DIR = os.path.join(dt.ROOT, "oracle", "_ref", "glue")
+GLUE_GPU = os.path.join(GLUE_DIR, "hdsdp_glue_run")
+GLUE_ORACLE = os.path.join(GLUE_DIR, "hdsdp_glue_run_oracle")
+
+
+def run_glue(binary, name, timeout=1200):
+    if not os.path.exists(binary):
+        pytest.skip("boundary shim not built (needs the reference sources at build time)")
+    inst, fix = dt.load_golden(name)
+    print(f"[glue] {os.path.basename(binary)} on {name}", flush=True)
+    res, y, log, out = mg.run_reference(inst, dict(fix["options"]), timeout=timeout, runner=binary)
+    assert "GLUE " in out, out[-2000:]
+    calls = json.loads(out.split("GLUE ", 1)[1].splitlines()[0])
+    assert res.get("retcode") == 0, out[-3000:]
+    r = types.SimpleNamespace(status=res["status"], iters=res["iters"], pobj=res["pobj"], dobj=res["dobj"], y=y,
+                              log=log)
+    return r, calls, out, fix
+
+
+def check_forwarded(r, calls, out):
+    """The cone and the Schur solver were the device's, for every iteration."""
+    assert "[glue] DENSE_SDP cone" in out and "Schur solver: GPU kind" in out, out[:3000]
+    assert calls["schur_builds"] >= r.iters
+    assert calls["kkt_factorizations"] >= r.iters
+    assert calls["kkt_solves"] >= 2 * r.iters
+    assert calls["factorizations"] > 0 and calls["ratio_tests"] > 0 and calls["updates"] > 0
+    assert calls["primal_recoveries"] >= 1
+
+
+@pytest.mark.parametrize("name", ["c1_gap1e-6", "maxcut_n60", "theta_n50", "dense_n40_m60"])
+def test_reference_host_over_oracle_abi(name):
+    r, calls, out, fix = run_glue(GLUE_ORACLE, name)
+    check_forwarded(r, calls, out)
+    dt.check_parity(fix, r, f"glue/oracle {name}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["c1_gap1e-6", "c2_gap1e-6"])
+def test_reference_host_over_device_abi(name):
+    """C1 and C2 through the reference's unmodified hdsdp_algo.c with the device slots."""
+    r, calls, out, fix = run_glue(GLUE_GPU, name)
+    check_forwarded(r, calls, out)
+    dt.check_parity(fix, r, f"glue/gpu {name}")
