@@ -43,6 +43,9 @@ constexpr int PK = 32;      // k slab = one MFMA k
 constexpr int PPAD = 8;     // row padding (bf16)
 constexpr int PFLUSH = 16;  // slabs between FP64 accumulations (k = 512)
 constexpr int PTHREADS = 512;
+constexpr int PSLAB = 2 * 3 * PT * (PK + PPAD);  // bf16 elements of one operand's double buffer
+constexpr int CPAD = 4;                          // row padding of the staged output tile (floats)
+static_assert((long) PT * (PT + CPAD) * 4 <= 2L * PSLAB * 2, "output staging must fit in the slab buffers");
 
 __device__ inline void split3(float x, __bf16 &a, __bf16 &b, __bf16 &c) {
     a = (__bf16) x;
@@ -77,9 +80,10 @@ __global__ __launch_bounds__(PTHREADS) void gemm_bf16p_kernel(int M, int N, int 
     const int Ke = (tri == 1) ? min(K, j0 + PT) : (tri == 2) ? min(K, i0 + PT) : K;
     X += (long) blockIdx.z * sX;
     Y += (long) blockIdx.z * sY;
-    // planes [buffer][term][row][k]
-    __shared__ __attribute__((aligned(16))) __bf16 As[2][3][PT][PK + PPAD];
-    __shared__ __attribute__((aligned(16))) __bf16 Bs[2][3][PT][PK + PPAD];
+    // planes [buffer][term][row][k]; the same memory stages the output tile at the end
+    __shared__ __attribute__((aligned(16))) __bf16 smem[2 * PSLAB];
+    auto As = reinterpret_cast<__bf16(*)[3][PT][PK + PPAD]>(smem);
+    auto Bs = reinterpret_cast<__bf16(*)[3][PT][PK + PPAD]>(smem + PSLAB);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wi = (wave & 3) * 32, wj = (wave >> 2) * 64;
     const int lr = lane & 15, lg = lane >> 4;
@@ -177,6 +181,64 @@ __global__ __launch_bounds__(PTHREADS) void gemm_bf16p_kernel(int M, int N, int 
         __syncthreads();
     }
     // epilogue: lane holds rows i0 + wi + 16 a + 4 lg + r of column j0 + wj + 16 b + lr
+    if (OUT == 2 || (OUT == 0 && beta == 0.0)) {
+        // Staged through LDS (Ct[column][row], FP32), then written column by column: a
+        // thread writes four consecutive rows of one column, so a wave's stores cover
+        // whole 128-row column segments instead of one element per column.
+        float(*Ct)[PT + CPAD] = reinterpret_cast<float(*)[PT + CPAD]>(smem);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int jl = wj + 16 * b + lr;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) Ct[jl][wi + 16 * a + 4 * lg + r] = (float) (alpha * acc64[a][b][r]);
+            }
+        __syncthreads();
+        const int il = (tid & 31) * 4, jt = tid >> 5;  // rows il .. il + 3, columns jt + 16 q
+        const int i = i0 + il;
+#pragma unroll
+        for (int q = 0; q < PT / 16; ++q) {
+            const int jl = jt + 16 * q, j = j0 + jl;
+            if (j >= N) continue;
+            const float4 v = *reinterpret_cast<const float4 *>(&Ct[jl][il]);
+            const float vv[4] = {v.x, v.y, v.z, v.w};
+            const bool full = i + 3 < M && (!LOWER || i >= j);
+            if constexpr (OUT == 2) {
+                __bf16 *C = static_cast<__bf16 *>(Cv) + (long) blockIdx.z * sC;
+                const long e = i + (long) j * ldc;
+                __bf16 p[3][4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) split3(vv[u], p[0][u], p[1][u], p[2][u]);
+                if (full && (e & 3) == 0 && ((psC & 3) == 0)) {
+#pragma unroll
+                    for (int t = 0; t < 3; ++t) {
+                        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+                        *reinterpret_cast<bf16x4 *>(C + e + t * psC) = (bf16x4){p[t][0], p[t][1], p[t][2], p[t][3]};
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (i + u < M && (!LOWER || i + u >= j)) {
+                            C[e + u] = p[0][u];
+                            C[e + u + psC] = p[1][u];
+                            C[e + u + 2 * psC] = p[2][u];
+                        }
+                }
+            } else {
+                float *C = static_cast<float *>(Cv) + (long) blockIdx.z * sC;
+                const long e = i + (long) j * ldc;
+                if (full && (e & 3) == 0) {
+                    *reinterpret_cast<float4 *>(C + e) = v;
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (i + u < M && (!LOWER || i + u >= j)) C[e + u] = vv[u];
+                }
+            }
+        }
+        return;
+    }
     if constexpr (OUT == 2) {
         __bf16 *C = static_cast<__bf16 *>(Cv) + (long) blockIdx.z * sC;
 #pragma unroll

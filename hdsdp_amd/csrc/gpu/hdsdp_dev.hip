@@ -1000,6 +1000,12 @@ hdsdp_retcode hdsdp_dev_load(hdsdp_dev *d, const hdsdp_coeff_view *cv) {
     if (c.cDSR1) DCALL(gather_up(c.rF + (size_t) c.nR1 * n, 1, n, nullptr));
     c.cR1F = c.rF + (size_t) c.nR1 * n;
     DCALL(dalloc(d, &c.r1w, c.nR1));
+    c.packW = nullptr;
+    c.trPart = nullptr;
+    if (c.nDense > 0) {
+        DCALL(dalloc(d, &c.packW, 2 * npk));
+        DCALL(dalloc(d, &c.trPart, (size_t) 2 * hdk::TR_SPLIT * c.nDense));
+    }
     {
         DCALL(dalloc(d, &c.posIdx, posIdx.size()));
         if (!posIdx.empty())

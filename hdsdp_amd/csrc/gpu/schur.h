@@ -40,7 +40,11 @@ struct DevCoeffs {
     int *smKind, *smNnz, *smRow, *smCol;  // (m + 1) x SMALL_K
     double *smVal;
     int pairSingle;   // every small constraint is one sparse entry and n <= PAIR1_MAXN
+    double *packW;    // 2 n(n+1)/2 scratch: packed buffers of the dense assembly and traces
+    double *trPart;   // 2 TR_SPLIT nDense partial traces of the dense coefficients
 };
+
+constexpr int TR_SPLIT = 4;  // k-parts of the dense trace products (summed in a fixed order)
 
 // target = sum_k coef[k] A_k (coef[0] multiplies C) + eye I on the lower triangle.
 // cCoef is coef[0] on the host (needed for a dense rank-one C).  work: n * nR1.

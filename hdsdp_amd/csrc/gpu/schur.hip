@@ -17,46 +17,56 @@ namespace hdk {
 
 // ---- buffer assembly -----------------------------------------------------------
 
-// target(i, j), i >= j: sum over dense coefficients of coef * packed value; zero elsewhere.
-// target(i, j) = sum_d coef[denseCon[d]] A_d(i, j) for i >= j, 0 above the diagonal.
-// Thread (i, j) walks the dense slots in order (the packed entries of one column are
-// contiguous, so a wave's loads are coalesced); the slot coefficients are staged in LDS
-// and eight slot loads are issued before they are summed, so the kernel streams the
-// packed coefficient matrices at HBM rate.
+// Dense part of a buffer: y = sum_d coef[denseCon[d]] A_d over the packed lower
+// entries e (one thread per entry pair e, e + 256 of a 512-entry chunk), the slots
+// summed in order, then the packed sum written out as the full lower triangle (zeros
+// above).  The packed columns of consecutive slots are npk apart; a wave's loads of one
+// slot are contiguous, and sixteen loads per thread are issued before they are summed,
+// so the kernel streams the packed coefficient matrices at HBM rate.
 constexpr int ASM_CHUNK = 512;
 
-__global__ __launch_bounds__(256) void assemble_dense_kernel(int n, int nDense, const int *denseCon,
-                                                             const double *dVal, const double *coef, double *target) {
+__global__ __launch_bounds__(256) void assemble_packed_kernel(long npk, int nDense, const int *denseCon,
+                                                              const double *dVal, const double *coef, double *y) {
     __shared__ double cs[ASM_CHUNK];
-    const int j = blockIdx.y;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (min(n, (int) ((blockIdx.x + 1) * blockDim.x)) <= j) {
-        // the whole block lies above the diagonal
-        if (i < n) target[i + (long) j * n] = 0.0;
-        return;
-    }
-    const bool act = i < n && i >= j;
-    const long npk = (long) n * (n + 1) / 2;
-    const double *src = dVal + (act ? (long) j * n - (long) j * (j - 1) / 2 + (i - j) : 0);
-    double v = 0.0;
+    const long e0 = blockIdx.x * 512L + threadIdx.x, e1 = e0 + 256;
+    const bool a0 = e0 < npk, a1 = e1 < npk;
+    double v0 = 0.0, v1 = 0.0;
     for (int d0 = 0; d0 < nDense; d0 += ASM_CHUNK) {
         const int cnt = min(ASM_CHUNK, nDense - d0);
         __syncthreads();
         for (int t = threadIdx.x; t < cnt; t += blockDim.x) cs[t] = coef[denseCon[d0 + t]];
         __syncthreads();
-        if (!act) continue;
-        const double *q = src + (long) d0 * npk;
+        const double *q = dVal + (long) d0 * npk;
         int t = 0;
         for (; t + 8 <= cnt; t += 8) {
-            double x[8];
+            double x0[8], x1[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) x[u] = q[(long) (t + u) * npk];
+            for (int u = 0; u < 8; ++u) {
+                x0[u] = a0 ? q[(long) (t + u) * npk + e0] : 0.0;
+                x1[u] = a1 ? q[(long) (t + u) * npk + e1] : 0.0;
+            }
 #pragma unroll
-            for (int u = 0; u < 8; ++u) v += cs[t + u] * x[u];
+            for (int u = 0; u < 8; ++u) {
+                v0 += cs[t + u] * x0[u];
+                v1 += cs[t + u] * x1[u];
+            }
         }
-        for (; t < cnt; ++t) v += cs[t] * q[(long) t * npk];
+        for (; t < cnt; ++t) {
+            if (a0) v0 += cs[t] * q[(long) t * npk + e0];
+            if (a1) v1 += cs[t] * q[(long) t * npk + e1];
+        }
     }
-    if (i < n) target[i + (long) j * n] = act ? v : 0.0;
+    if (a0) y[e0] = v0;
+    if (a1) y[e1] = v1;
+}
+
+// target(i, j) = y(packed (i, j)) for i >= j, 0 above (packed column j starts at
+// j n - j (j - 1) / 2)
+__global__ void unpack_packed_kernel(int n, const double *y, double *target) {
+    const int j = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    target[i + (long) j * n] = (i >= j) ? y[(long) j * n - (long) j * (j - 1) / 2 + (i - j)] : 0.0;
 }
 
 __global__ void scatter_kernel(int nPos, const long *posIdx, const int *posBeg, const int *entCon,
@@ -96,7 +106,14 @@ void assemble_buffer(hipStream_t s, const DevCoeffs &c, const double *coef, doub
                      double eye, double *work, double *splitWork, size_t splitLen) {
     const int n = c.n;
     dim3 g((n + 255) / 256, n);
-    assemble_dense_kernel<<<g, 256, 0, s>>>(n, c.nDense, c.denseCon, c.dVal, coef, target);
+    const long npk = (long) n * (n + 1) / 2;
+    if (c.nDense > 0) {
+        assemble_packed_kernel<<<(unsigned) ((npk + 511) / 512), 256, 0, s>>>(npk, c.nDense, c.denseCon, c.dVal,
+                                                                               coef, c.packW);
+        unpack_packed_kernel<<<g, 256, 0, s>>>(n, c.packW, target);
+    } else {
+        fill(s, target, (long) n * n, 0.0);
+    }
     if (c.nPos > 0) scatter_kernel<<<(c.nPos + 255) / 256, 256, 0, s>>>(c.nPos, c.posIdx, c.posBeg, c.entCon,
                                                                          c.entVal, coef, target);
     if (c.nR1 > 0) {
@@ -130,8 +147,82 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
+// Dense coefficients: <A_d, B> = sum over packed lower entries e of A_d(e) w(e) B(e),
+// w = 1 on the diagonal and 2 below it.  The weighted packed B (and B2) is formed once,
+// then one pass over the packed coefficients computes TR_CPB traces per workgroup over
+// one of TR_SPLIT k-parts (a multi-column dot product streamed at HBM rate); the parts
+// are added in a fixed order.
+constexpr int TR_CPB = 4;
+
+__global__ void pack_weighted_kernel(int n, const double *B1, const double *B2, double *w1, double *w2) {
+    const int j = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || i < j) return;
+    const long e = (long) j * n - (long) j * (j - 1) / 2 + (i - j);
+    const double w = (i == j) ? 1.0 : 2.0;
+    w1[e] = w * B1[i + (long) j * n];
+    if (B2) w2[e] = w * B2[i + (long) j * n];
+}
+
+__global__ __launch_bounds__(256) void dense_trace_kernel(long npk, int nDense, const double *dVal, const double *w1,
+                                                          const double *w2, double *part1, double *part2) {
+    __shared__ double red[2 * TR_CPB][4];
+    const int d0 = blockIdx.x * TR_CPB, split = blockIdx.y, tid = threadIdx.x;
+    const long chunk = ((npk + TR_SPLIT - 1) / TR_SPLIT + 255) / 256 * 256;
+    const long ks = split * chunk, ke = min(npk, ks + chunk);
+    double a1[TR_CPB], a2[TR_CPB];
+    const double *A[TR_CPB];
+#pragma unroll
+    for (int c = 0; c < TR_CPB; ++c) {
+        a1[c] = 0.0;
+        a2[c] = 0.0;
+        A[c] = dVal + (long) min(d0 + c, nDense - 1) * npk;
+    }
+    for (long e = ks + tid; e < ke; e += 256) {
+        const double b1 = w1[e], b2 = w2 ? w2[e] : 0.0;
+#pragma unroll
+        for (int c = 0; c < TR_CPB; ++c) {
+            const double av = A[c][e];
+            a1[c] += av * b1;
+            a2[c] += av * b2;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < TR_CPB; ++c) {
+        a1[c] = wave_sum(a1[c]);
+        a2[c] = wave_sum(a2[c]);
+        if ((tid & 63) == 0) {
+            red[c][tid >> 6] = a1[c];
+            red[TR_CPB + c][tid >> 6] = a2[c];
+        }
+    }
+    __syncthreads();
+    if (tid < TR_CPB && d0 + tid < nDense) {
+        const int c = tid;
+        part1[(long) split * nDense + d0 + c] = red[c][0] + red[c][1] + red[c][2] + red[c][3];
+        part2[(long) split * nDense + d0 + c] =
+            red[TR_CPB + c][0] + red[TR_CPB + c][1] + red[TR_CPB + c][2] + red[TR_CPB + c][3];
+    }
+}
+
+__global__ void dense_trace_finish_kernel(int nDense, const int *denseCon, int k0, int cnt, const double *part1,
+                                          const double *part2, double s2, double *out1, double *out2) {
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= nDense) return;
+    const int k = denseCon[d];
+    if (k < k0 || k >= k0 + cnt) return;
+    double t1 = 0.0, t2 = 0.0;
+    for (int p = 0; p < TR_SPLIT; ++p) {
+        t1 += part1[(long) p * nDense + d];
+        t2 += part2[(long) p * nDense + d];
+    }
+    out1[k - k0] = t1;
+    if (out2) out2[k - k0] = s2 * t2;
+}
+
 // out1[k] = <A_k, B1>, out2[k] = s2 <A_k, B2> (B2 may be NULL) for coefficients k0 .. k0 + cnt - 1,
-// written to out[k - k0].  One workgroup of 256 threads per coefficient.
+// written to out[k - k0].  One workgroup of 256 threads per coefficient; dense
+// coefficients are left to dense_trace_kernel.
 __global__ __launch_bounds__(256) void trace_kernel(int n, DevCoeffs c, int k0, const double *B1, const double *B2,
                                                     double s2, double *out1, double *out2, int withR1) {
     __shared__ double red1[4], red2[4];
@@ -139,6 +230,7 @@ __global__ __launch_bounds__(256) void trace_kernel(int n, DevCoeffs c, int k0, 
     const int tid = threadIdx.x;
     const int t = c.ctype[k];
     if (t == COEFF_DSR1 && !withR1) return;  // left to r1_quadforms
+    if (t == COEFF_DENSE) return;            // left to dense_trace_kernel
     double a1 = 0.0, a2 = 0.0;
     if (t == COEFF_SPARSE) {
         for (int e = c.cBeg[k] + tid; e < c.cBeg[k + 1]; e += 256) {
@@ -159,19 +251,6 @@ __global__ __launch_bounds__(256) void trace_kernel(int n, DevCoeffs c, int k0, 
         }
         a1 *= c.csign[k];
         a2 *= c.csign[k];
-    } else if (t == COEFF_DENSE) {
-        const double *A = c.dVal + (long) c.dSlot[k] * ((long) n * (n + 1) / 2);
-        long colStart = 0;
-        for (int j = 0; j < n; ++j) {
-            for (int i = j + tid; i < n; i += 256) {
-                double av = A[colStart + (i - j)];
-                double w = (i == j) ? av : 2.0 * av;
-                long idx = i + (long) j * n;
-                a1 += w * B1[idx];
-                if (B2) a2 += w * B2[idx];
-            }
-            colStart += n - j;
-        }
     } else if (t == COEFF_DSR1) {
         // f^T B f with B symmetric: column i of B is read contiguously
         const double *f = c.rF + (long) c.rSlot[k] * n;
@@ -206,6 +285,16 @@ void traces(hipStream_t s, const DevCoeffs &c, int k0, int cnt, const double *B1
             double *out1, double *out2, bool withR1) {
     if (cnt <= 0) return;
     trace_kernel<<<cnt, 256, 0, s>>>(c.n, c, k0, B1, B2, s2, out1, out2, withR1 ? 1 : 0);
+    if (c.nDense > 0) {
+        const int n = c.n;
+        const long npk = (long) n * (n + 1) / 2;
+        double *w1 = c.packW, *w2 = B2 ? c.packW + npk : nullptr;
+        pack_weighted_kernel<<<dim3((n + 255) / 256, n), 256, 0, s>>>(n, B1, B2, w1, w2);
+        dense_trace_kernel<<<dim3((c.nDense + TR_CPB - 1) / TR_CPB, TR_SPLIT), 256, 0, s>>>(
+            npk, c.nDense, c.dVal, w1, w2, c.trPart, c.trPart + (long) TR_SPLIT * c.nDense);
+        dense_trace_finish_kernel<<<(c.nDense + 255) / 256, 256, 0, s>>>(
+            c.nDense, c.denseCon, k0, cnt, c.trPart, c.trPart + (long) TR_SPLIT * c.nDense, s2, out1, out2);
+    }
 }
 
 // ---- dense rank-one coefficients through GEMMs ------------------------------------------

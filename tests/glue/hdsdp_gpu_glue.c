@@ -26,10 +26,8 @@
  * keeps (kktMatElem) receives only M's diagonal from each build, so that the
  * reference's own bound-cone terms (interface/hdsdp_conic_bound.c:224-229) and
  * HKKTRegularize (interface/hdsdp_schur.c:348-373) act on the true diagonal; the
- * GPU kind's numeric slot then sends each diagonal's change back to the device
- * as the increment whose addition to the built diagonal d rounds to the host
- * value x (checked on the host by the same addition), so the device diagonal
- * becomes x bit for bit.
+ * GPU kind's numeric slot then sets the device diagonal to the host's, bit for
+ * bit, with two additions (-d, then x).
  *
  * Supported: one DENSE_SDP cone, a dense Schur matrix, the dual-scaling
  * iterations (KKT types INFEASIBLE and CORRECTOR).  PSDP (KKT_TYPE_PRIMAL) and the
@@ -497,50 +495,33 @@ static hdsdp_retcode gk_symbolic( void *chol, int *beg, int *idx ) {
     return HDSDP_RETCODE_OK;
 }
 
-/* The increment that takes the device diagonal d to the host value x: the device
- * adds it in one IEEE double addition (M_ii += delta), which the host repeats to
- * check; x - d rounded is moved by an ulp at a time until d + delta rounds to x.
- * Such a delta exists for every x >= d, as the rounded sum steps through
- * consecutive doubles while delta does. */
-static int exact_increment( double d, double x, double *delta ) {
-    double t = x - d;
-    for ( int k = 0; k < 8; ++k ) {
-        volatile double sum = d + t;
-        if ( sum == x ) {
-            *delta = t;
-            return 1;
-        }
-        t = nextafter(t, sum < x ? HUGE_VAL : -HUGE_VAL);
-    }
-    return 0;
-}
-
-/* conjGradLinSolverNumeric (768-778): the host diagonal's change goes back to M on
- * the device, then the preconditioner is built there. */
+/* conjGradLinSolverNumeric (768-778): the host diagonal goes back to M on the
+ * device, then the preconditioner is built there.  The device only adds to its
+ * diagonal (M_ii += v_i, one IEEE addition), so the diagonal is first cleared by
+ * adding -d (d + (-d) = 0 exactly) and then set by adding x (0 + x = x): the device
+ * diagonal equals the host's bit for bit. */
 static hdsdp_retcode gk_numeric_dev( glue_kkt *k, double *elem ) {
     glue_cone *g = k->g;
     const int m = k->nCol;
     int changed = 0;
-    for ( int i = 0; i < m; ++i ) {
-        const double x = elem[(size_t) i * m + i], d = g->mDiag[i];
-        k->delta[i] = 0.0;
-        if ( x == d ) continue;
-        if ( !exact_increment(d, x, &k->delta[i]) ) {
-            fprintf(stderr, "[glue] Schur diagonal %d: no increment takes %.17g to %.17g\n", i, d, x);
-            return HDSDP_RETCODE_FAILED;
-        }
-        changed = 1;
-    }
+    for ( int i = 0; i < m; ++i ) changed |= ( elem[(size_t) i * m + i] != g->mDiag[i] );
     hdsdp_retcode retcode = HDSDP_RETCODE_OK;
     if ( changed ) {
         gCalls[7] += 1;
+        for ( int i = 0; i < m; ++i ) k->delta[i] = -g->mDiag[i];
         GCALL(hdsdp_dev_kkt_add_diag(g->dev, k->delta));
-        for ( int i = 0; i < m; ++i ) g->mDiag[i] = elem[(size_t) i * m + i];
+        for ( int i = 0; i < m; ++i ) g->mDiag[i] = k->delta[i] = elem[(size_t) i * m + i];
+        GCALL(hdsdp_dev_kkt_add_diag(g->dev, k->delta));
     }
     gCalls[4] += 1;
     GCALL(hdsdp_dev_kkt_factorize(g->dev));
 exit_cleanup:
     return retcode;
+}
+
+static hdsdp_retcode gk_numeric( void *chol, int *beg, int *idx, double *elem ) {
+    (void) beg; (void) idx;
+    return gk_numeric_dev((glue_kkt *) chol, elem);
 }
 
 static hdsdp_retcode gk_psd_check( void *chol, int *beg, int *idx, double *elem, int *isPsd ) {
@@ -550,11 +531,6 @@ static hdsdp_retcode gk_psd_check( void *chol, int *beg, int *idx, double *elem,
 
 static void gk_noop_solve( void *chol, int nRhs, double *rhs, double *sol ) {
     (void) chol; (void) nRhs; (void) rhs; (void) sol;
-}
-
-static hdsdp_retcode gk_numeric( void *chol, int *beg, int *idx, double *elem ) {
-    (void) beg; (void) idx;
-    return gk_numeric_dev((glue_kkt *) chol, elem);
 }
 
 /* conjGradLinSolverSolveN (999-1016): sol = NULL overwrites rhs */

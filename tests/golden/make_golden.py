@@ -42,6 +42,7 @@ RUNNER_DENSE_FMA = os.path.join(ROOT, "oracle", "_ref", "dense_fma", "hdsdp_ref_
 #             PARDISO (interface/hdsdp_conic_sdp.c:65-97); the dense-dual build
 #             (oracle/_ref/dense) is run as well and its result kept beside the default one
 #   "heavy":  the spread runs use the default thread count only (one thread takes minutes)
+#   "medium": the spread runs use both thread counts over the first three MKL paths
 #   "dense_primary": the primary result comes from the dense-dual build (the default build
 #             stops with a segmentation fault on this instance: its primal method symmetrises
 #             the dual step as a dense n x n array while PARDISO holds it sparse,
@@ -61,7 +62,7 @@ CASES = {
     # the dense (C3) and rank-one (C4) families at sizes the CPU reference solves in seconds
     "dense_n120_m240": (lambda: I.dense_random(120, 240, 33), {"relopt": 1e-6, "absopt": 1e30}, set()),
     "rankone_n60_m500": (lambda: I.rank_one(60, 500, 5, 45, form="csc"), {"relopt": 1e-6, "absopt": 1e30}, set()),
-    "dense_n200_m400": (lambda: I.dense_random(200, 400, 34), {"relopt": 1e-6, "absopt": 1e30}, {"heavy"}),
+    "dense_n200_m400": (lambda: I.dense_random(200, 400, 34), {"relopt": 1e-6, "absopt": 1e30}, {"medium"}),
     "rankone_n200_m2000": (lambda: I.rank_one(200, 2000, 20, 4, form="csc"), {"relopt": 1e-6, "absopt": 1e30},
                            {"heavy"}),
     # the primal method on (it starts once the dual gap is below 10% and ends the solve)
@@ -134,7 +135,7 @@ def spread_runs(inst, opts, flags, primary):
     elif "sparse" in flags:
         builds += [("dense", RUNNER_DENSE), ("dense_fma", RUNNER_DENSE_FMA)]
     threads = [None] if "heavy" in flags else [None, 1]
-    paths = CBWR_PATHS[:3] if "heavy" in flags else CBWR_PATHS
+    paths = CBWR_PATHS[:3] if ("heavy" in flags or "medium" in flags) else CBWR_PATHS
     runs, results = [], []
     for bname, runner in builds:
         for th in threads:
