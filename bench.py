@@ -276,19 +276,29 @@ def main():
         }
         sc_ms, sc_cnt = prof["schur"]
         gr_ms, gr_cnt = prof["gram"]
+        g16_ms, g16_cnt = prof.get("gram16", [0.0, 0])
         if sflops is not None and sc_cnt:
-            prod = 1 if p == 0 else BF16_PRODUCTS
-            peak = FP64_PEAK_TFLOPS if p == 0 else BF16_PEAK_TFLOPS
+            # the bf16 figures come from the builds made of bf16 split products only (a
+            # mixed solve makes its late, ill-conditioned builds in FP64)
+            bf16 = p != 0 and g16_cnt > 0
+            prod = BF16_PRODUCTS if bf16 else 1
+            peak = BF16_PEAK_TFLOPS if bf16 else FP64_PEAK_TFLOPS
+            g_ms, g_cnt = (g16_ms, g16_cnt) if bf16 else (gr_ms, gr_cnt)
             s["schur_build"] = {
                 "algorithmic_flops": sflops,
                 "avg_ms": sc_ms / sc_cnt,
                 "tflops_fp64_equivalent": sflops / (sc_ms / sc_cnt / 1e3) / 1e12,
-                "gram_avg_ms": gr_ms / max(gr_cnt, 1),
+                "gram_avg_ms": g_ms / max(g_cnt, 1),
                 "mfma_products_per_flop": prod,
-                "mfma_tflops": prod * sflops / (gr_ms / max(gr_cnt, 1) / 1e3) / 1e12 if gr_cnt else None,
+                "mfma_tflops": prod * sflops / (g_ms / max(g_cnt, 1) / 1e3) / 1e12 if g_cnt else None,
                 "peak_tflops": peak,
                 "builds": sc_cnt,
             }
+            if p != 0:
+                s["schur_build"]["bf16_builds"] = g16_cnt
+                s["schur_build"]["fp64_builds"] = sc_cnt - g16_cnt
+                if gr_cnt > g16_cnt:
+                    s["schur_build"]["fp64_gram_avg_ms"] = (gr_ms - g16_ms) / (gr_cnt - g16_cnt)
             s["schur_build"]["frac"] = (s["schur_build"]["mfma_tflops"] or 0.0) / peak
         mf_ms, mf_cnt = prof["mfactor"]
         if mf_cnt:
@@ -303,7 +313,8 @@ def main():
     # roofline of the dominant kernel of the headline path
     if sflops is not None and "schur_build" in hs and hprof["gram"][1]:
         sb = hs["schur_build"]
-        kern = "gemm_bf16s_tn_kernel (bf16 split Gram products)" if head == "mixed" else \
+        kern = "gemm_bf16p_kernel (bf16 plane products of the dense Schur build: V = A L^-T, " \
+            "T = L^-1 V, <T_p, T_q>)" if sb["mfma_products_per_flop"] > 1 else \
             "gemm_kernel / gemm_big_kernel (FP64 Gram products)"
         roof = {
             "bound": "mfma", "kernel": kern, "achieved": sb["mfma_tflops"], "peak": sb["peak_tflops"],

@@ -82,7 +82,7 @@ _API = {
 }
 
 # Device-time profile slots (HDSDP_PROF_* in include/hdsdp_gpu.h)
-PROF_SLOTS = ("schur", "pairs", "mfactor", "cg", "sfactor", "lanczos", "buffer", "primal", "gram")
+PROF_SLOTS = ("schur", "pairs", "mfactor", "cg", "sfactor", "lanczos", "buffer", "primal", "gram", "gram16")
 
 # Entry points every build of the library exports (include/hdsdp.h, include/hdsdp_gpu.h).
 EXPORTED = tuple(_API)

@@ -51,16 +51,19 @@ def centre_at(lib, inst: I.SDPInstance, mu: float, y0: np.ndarray | None = None,
 
 
 def fd_check(lib, inst: I.SDPInstance, n_dirs: int, *, seed: int = 0, h: float = 1e-6, relopt: float = 1e-8,
-             log=None, **solve_opts):
+             log=None, noise: bool = False, **solve_opts):
     """Adjoint of the centred map against central finite differences.
 
     Solves ``inst`` to ``relopt``, centres at the final mu and takes the gradient of
     L = ||y_mu - y_t||^2 (y_t seeded normal) by HDSDPAdjoint.  Along ``n_dirs`` seeded
     random directions (db, dC), dC symmetric, it compares <dL/db, db> + <dL/dC, dC>
     with (L(+h) - L(-h)) / 2h, where L(+-h) centres a solve of the perturbed data at
-    the same mu.  ``log(k, err)`` is called after each direction.  Returns a dict:
-    relative errors per direction, mu, the gradients, y_mu and the seconds of the
-    adjoint call."""
+    the same mu.  With ``noise`` the quotient is also taken with step 2h: its distance
+    from the step-h quotient measures that direction's finite-difference error (the
+    truncation term grows fourfold from h to 2h, the rounding term of the centred
+    points halves).  ``log(k, err)`` is called after each direction.  Returns a dict:
+    relative errors per direction, mu, the gradients, y_mu, the seconds of the adjoint
+    call, and with ``noise`` the relative distances |fd(h) - fd(2h)| / |fd(h)|."""
     import time
 
     rng = np.random.default_rng(seed)
@@ -74,22 +77,31 @@ def fd_check(lib, inst: I.SDPInstance, n_dirs: int, *, seed: int = 0, h: float =
     finally:
         sv.close()
     C0 = inst.dense_coeff(0)
-    errs = []
+    errs, noises = [], []
+
+    def quotient(step, dbv, dCv):
+        Lv = []
+        for sgn in (1.0, -1.0):
+            ip = I.with_data(inst, b=inst.b + sgn * step * dbv, C=C0 + sgn * step * dCv)
+            yp = centre_at(lib, ip, mu, **opts)
+            Lv.append(float(np.sum((yp - yt) ** 2)))
+        return (Lv[0] - Lv[1]) / (2.0 * step)
+
     for _ in range(n_dirs):
         dbv = rng.standard_normal(inst.m)
         dCv = rng.standard_normal((inst.n, inst.n))
         dCv = 0.5 * (dCv + dCv.T)
-        Lv = []
-        for sgn in (1.0, -1.0):
-            ip = I.with_data(inst, b=inst.b + sgn * h * dbv, C=C0 + sgn * h * dCv)
-            yp = centre_at(lib, ip, mu, **opts)
-            Lv.append(float(np.sum((yp - yt) ** 2)))
-        fd = (Lv[0] - Lv[1]) / (2.0 * h)
+        fd = quotient(h, dbv, dCv)
         an = float(db @ dbv + np.sum(dC * dCv))
         errs.append(abs(fd - an) / max(abs(fd), 1e-300))
+        if noise:
+            noises.append(abs(fd - quotient(2.0 * h, dbv, dCv)) / max(abs(fd), 1e-300))
         if log:
             log(len(errs), errs[-1])
-    return {"errs": errs, "mu": mu, "dLdb": db, "dLdC": dC, "y": y, "adjoint_s": t_adj}
+    out = {"errs": errs, "mu": mu, "dLdb": db, "dLdC": dC, "y": y, "adjoint_s": t_adj}
+    if noise:
+        out["fd_noise"] = noises
+    return out
 
 
 class SDPDual(torch.autograd.Function):

@@ -23,6 +23,13 @@ inline long plane_ld(long n) { return (n + 7) / 8 * 8; }
 void gemm_bf16p(hipStream_t s, bool lower, int M, int N, int K, double alpha, const __bf16 *X, int ldx, long psX,
                 long sX, const __bf16 *Y, int ldy, long psY, long sY, double beta, void *C, int outKind, int ldc,
                 long psC, long sC, int batch = 1, int tri = 0);
+// Symmetric C = alpha X^T Y (n x n, only i >= j formed) written as three bf16 planes,
+// packed: the strictly lower part column by column (entry (i, j), i > j, at
+// j n - j (j + 1) / 2 + i - j - 1 of off; batch stride sOff, plane stride psOff) and
+// the diagonal (entry i of dg; batch stride sDg, plane stride psDg).  Operands as above.
+void gemm_bf16p_packed(hipStream_t s, int n, int K, double alpha, const __bf16 *X, int ldx, long psX, long sX,
+                       const __bf16 *Y, int ldy, long psY, long sY, __bf16 *off, long sOff, long psOff, __bf16 *dg,
+                       long sDg, long psDg, int batch = 1, int tri = 0);
 // planes (ld, plane stride ps) of a rows x cols matrix, rows..ld-1 zero
 void split_matrix(hipStream_t s, int rows, int cols, const double *A, long lda, __bf16 *P, int ld, long ps);
 void split_matrix_f(hipStream_t s, int rows, int cols, const float *A, long lda, __bf16 *P, int ld, long ps);
@@ -30,9 +37,5 @@ void split_matrix_f(hipStream_t s, int rows, int cols, const float *A, long lda,
 // slot0 .. slot0 + cnt - 1; the padding rows are left as they are (zero)
 void unpack_dense_split(hipStream_t s, const DevCoeffs &c, int slot0, int cnt, __bf16 *out, int ldn, long ps,
                         long stride);
-// T_p (cnt FP32 matrices n x n, lower valid) -> planes of the strictly lower part packed
-// by columns (column p at off + p ldo) and of the diagonal (dg + p ldd)
-void pack_lower_split(hipStream_t s, int n, int cnt, const float *Tin, long strideIn, __bf16 *off, long ldo, long pso,
-                      __bf16 *dg, int ldd, long psd);
 
 }  // namespace hdk

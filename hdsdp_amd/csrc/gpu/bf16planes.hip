@@ -58,13 +58,17 @@ __device__ inline void split3(float x, __bf16 &a, __bf16 &b, __bf16 &c) {
 // ends at its last column or row).  With tri == 2 on a 2-D grid the tile row is rotated
 // by the other grid coordinates, so that every XCD gets every row (the work of a tile
 // grows with its row).  lin: blockIdx.x runs over the lower tiles only.
-// OUT: 0 = FP32 C, 1 = FP64 C, 2 = three bf16 planes of C (at C + t * psC, ld ldc).
+// OUT: 0 = FP32 C, 1 = FP64 C, 2 = three bf16 planes of C (at C + t * psC, ld ldc),
+// 3 = three bf16 planes of a symmetric C (M = N = n, LOWER) packed: the strictly lower
+// part column by column (entry (i, j), i > j, at j n - j (j + 1) / 2 + i - j - 1 of C,
+// batch stride sC, plane stride psC) and the diagonal (entry i of Dg, batch stride sDg,
+// plane stride psDg).
 template <int OUT, bool LOWER>
 __global__ __launch_bounds__(PTHREADS) void gemm_bf16p_kernel(int M, int N, int K, double alpha,
                                                               const __bf16 *X, int ldx, long psX, long sX,
                                                               const __bf16 *Y, int ldy, long psY, long sY,
                                                               double beta, void *Cv, int ldc, long psC, long sC,
-                                                              int lin, int tri) {
+                                                              int lin, int tri, __bf16 *Dg, long sDg, long psDg) {
     int bx = blockIdx.x, by = blockIdx.y;
     if (!lin && tri == 2) bx = (int) ((blockIdx.x + blockIdx.y + blockIdx.z) % gridDim.x);
     if (lin) {
@@ -181,6 +185,40 @@ __global__ __launch_bounds__(PTHREADS) void gemm_bf16p_kernel(int M, int N, int 
         __syncthreads();
     }
     // epilogue: lane holds rows i0 + wi + 16 a + 4 lg + r of column j0 + wj + 16 b + lr
+    if constexpr (OUT == 3) {
+        // staged through LDS, then written one row per lane: a wave's stores of one
+        // column are consecutive packed entries
+        float(*Ct)[PT + CPAD] = reinterpret_cast<float(*)[PT + CPAD]>(smem);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int jl = wj + 16 * b + lr;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) Ct[jl][wi + 16 * a + 4 * lg + r] = (float) (alpha * acc64[a][b][r]);
+            }
+        __syncthreads();
+        __bf16 *off = static_cast<__bf16 *>(Cv) + (long) blockIdx.z * sC;
+        __bf16 *dg = Dg + (long) blockIdx.z * sDg;
+        const int il = tid & (PT - 1), jq = tid / PT, i = i0 + il;
+        for (int jl = jq; jl < PT; jl += PTHREADS / PT) {
+            const int j = j0 + jl;
+            if (j >= N || i >= M || i < j) continue;
+            __bf16 p0, p1, p2;
+            split3(Ct[jl][il], p0, p1, p2);
+            if (i > j) {
+                const long e = (long) j * M - (long) j * (j + 1) / 2 + (i - j - 1);
+                off[e] = p0;
+                off[e + psC] = p1;
+                off[e + 2 * psC] = p2;
+            } else {
+                dg[i] = p0;
+                dg[i + psDg] = p1;
+                dg[i + 2 * psDg] = p2;
+            }
+        }
+        return;
+    }
     if (OUT == 2 || (OUT == 0 && beta == 0.0)) {
         // Staged through LDS (Ct[column][row], FP32), then written column by column: a
         // thread writes four consecutive rows of one column, so a wave's stores cover
@@ -287,17 +325,37 @@ __global__ __launch_bounds__(PTHREADS) void gemm_bf16p_kernel(int M, int N, int 
 template <int OUT>
 void launch_p(hipStream_t s, bool lower, int M, int N, int K, double alpha, const __bf16 *X, int ldx, long psX,
               long sX, const __bf16 *Y, int ldy, long psY, long sY, double beta, void *C, int ldc, long psC, long sC,
-              int batch, int tri) {
+              int batch, int tri, __bf16 *Dg = nullptr, long sDg = 0, long psDg = 0) {
     const int mt = (M + PT - 1) / PT, nt = (N + PT - 1) / PT;
     const int lin = (lower && mt == nt) ? 1 : 0;
     dim3 grid(lin ? mt * (mt + 1) / 2 : mt, lin ? 1 : nt, batch);
     if (lower)
         gemm_bf16p_kernel<OUT, true><<<grid, PTHREADS, 0, s>>>(M, N, K, alpha, X, ldx, psX, sX, Y, ldy, psY, sY, beta,
-                                                               C, ldc, psC, sC, lin, tri);
-    else
+                                                               C, ldc, psC, sC, lin, tri, Dg, sDg, psDg);
+    else if constexpr (OUT != 3)
         gemm_bf16p_kernel<OUT, false><<<grid, PTHREADS, 0, s>>>(M, N, K, alpha, X, ldx, psX, sX, Y, ldy, psY, sY,
-                                                                beta, C, ldc, psC, sC, 0, tri);
+                                                                beta, C, ldc, psC, sC, 0, tri, Dg, sDg, psDg);
 }
+
+}  // namespace
+
+void gemm_bf16p_packed(hipStream_t s, int n, int K, double alpha, const __bf16 *X, int ldx, long psX, long sX,
+                       const __bf16 *Y, int ldy, long psY, long sY, __bf16 *off, long sOff, long psOff, __bf16 *dg,
+                       long sDg, long psDg, int batch, int tri) {
+    if (n <= 0 || batch <= 0) return;
+    const bool aligned = K % 8 == 0 && ldx % 8 == 0 && ldy % 8 == 0 && psX % 8 == 0 && psY % 8 == 0 &&
+                         sX % 8 == 0 && sY % 8 == 0 && ((uintptr_t) X % 16) == 0 && ((uintptr_t) Y % 16) == 0 &&
+                         K <= ldx && K <= ldy;
+    if (!aligned) {
+        fprintf(stderr, "[hdsdp_amd] gemm_bf16p_packed: operand layout not 8-element aligned (K %d ldx %d ldy %d); "
+                        "product skipped\n", K, ldx, ldy);
+        return;
+    }
+    launch_p<3>(s, true, n, n, K, alpha, X, ldx, psX, sX, Y, ldy, psY, sY, 0.0, off, 0, psOff, sOff, batch, tri, dg,
+                sDg, psDg);
+}
+
+namespace {
 
 // Three bf16 planes of a rows x cols FP64 (or FP32) matrix, rows rows..ld-1 of the planes
 // zero (the k padding of a product operand).
@@ -357,34 +415,6 @@ __global__ __launch_bounds__(256) void unpack_dense_split_kernel(int n, const do
     }
 }
 
-// Strictly lower part of each symmetric T_p (n x n FP32, lower valid) packed column by
-// column into planes of off (column p at p * ldo, entry j n - j (j + 1) / 2 + (i - j - 1)
-// for i > j) and its diagonal into planes of dg (column p at p * ldd).
-__global__ void pack_lower_split_kernel(int n, const float *Tin, long strideIn, __bf16 *off, long ldo, long pso,
-                                        __bf16 *dg, int ldd, long psd) {
-    const int p = blockIdx.y;
-    const long nn = (long) n * n;
-    const float *src = Tin + p * strideIn;
-    for (long e = blockIdx.x * (long) blockDim.x + threadIdx.x; e < nn; e += (long) gridDim.x * blockDim.x) {
-        const long i = e % n, j = e / n;
-        if (i < j) continue;
-        __bf16 p0, p1, p2;
-        split3(src[e], p0, p1, p2);
-        __bf16 *o;
-        long ps;
-        if (i > j) {
-            o = off + p * ldo + j * n - j * (j + 1) / 2 + (i - j - 1);
-            ps = pso;
-        } else {
-            o = dg + (long) p * ldd + i;
-            ps = psd;
-        }
-        o[0] = p0;
-        o[ps] = p1;
-        o[2 * ps] = p2;
-    }
-}
-
 }  // namespace
 
 void gemm_bf16p(hipStream_t s, bool lower, int M, int N, int K, double alpha, const __bf16 *X, int ldx, long psX,
@@ -427,10 +457,5 @@ void unpack_dense_split(hipStream_t s, const DevCoeffs &c, int slot0, int cnt, _
     unpack_dense_split_kernel<<<dim3(nt * (nt + 1) / 2, cnt), 256, 0, s>>>(c.n, c.dVal, slot0, out, ldn, ps, stride);
 }
 
-void pack_lower_split(hipStream_t s, int n, int cnt, const float *Tin, long strideIn, __bf16 *off, long ldo, long pso,
-                      __bf16 *dg, int ldd, long psd) {
-    if (cnt <= 0) return;
-    pack_lower_split_kernel<<<dim3(1024, cnt), 256, 0, s>>>(n, Tin, strideIn, off, ldo, pso, dg, ldd, psd);
-}
 
 }  // namespace hdk

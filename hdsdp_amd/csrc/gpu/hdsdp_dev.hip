@@ -53,6 +53,7 @@ struct hdsdp_dev {
     // kernel): L^-T, a batch of A_p and of V_p = A_p L^-T (ld plane_ld(n)), the packed T_p
     // (ld K1p, a multiple of 32, entries K1..K1p zero) and their diagonals
     __bf16 *LiTp, *Dp, *Vp, *Tp, *Tdgp;
+    __bf16 *Dall;  // planes of every dense constraint, split once at set-up (or null)
     long K1p, chunkP;
     int planes;
     // adaptive precision: the Gram blocks of M come from bf16 split products while
@@ -686,16 +687,19 @@ void dense_gram_mixed(hdsdp_dev *d) {
     hdk::split_matrix(d->st, n, n, d->LiT[0], n, d->LiTp, ldn, nnp);
     for (int b0 = 0; b0 < r; b0 += d->gramBatch) {
         const int cnt = std::min(d->gramBatch, r - b0);
-        hdk::unpack_dense_split(d->st, d->c, b0, cnt, d->Dp, ldn, bps, nnp);
+        // A_p planes: resident (split at set-up) or split for this batch
+        const __bf16 *Ap = d->Dall ? d->Dall + (size_t) b0 * nnp : d->Dp;
+        const long psA = d->Dall ? nnp * r : bps;
+        if (!d->Dall) hdk::unpack_dense_split(d->st, d->c, b0, cnt, d->Dp, ldn, bps, nnp);
         // V = A_p L^-T: column j of L^-T is zero below row j (tri 1); written as planes
-        hdk::gemm_bf16p(d->st, false, n, n, ldn, 1.0, d->Dp, ldn, bps, nnp, d->LiTp, ldn, nnp, 0, 0.0, d->Vp,
+        hdk::gemm_bf16p(d->st, false, n, n, ldn, 1.0, Ap, ldn, psA, nnp, d->LiTp, ldn, nnp, 0, 0.0, d->Vp,
                         hdk::BF16P_OUT_PLANES, ldn, bps, nnp, cnt, 1);
-        // lower triangle of T_p = (L^-T)^T V only (symmetric); column i of the left
-        // operand L^-T is zero below row i (tri 2)
-        hdk::gemm_bf16p(d->st, true, n, n, ldn, 1.0, d->LiTp, ldn, nnp, 0, d->Vp, ldn, bps, nnp, 0.0, d->Dbuff,
-                        hdk::BF16P_OUT_F32, n, 0, nn, cnt, 2);
-        hdk::pack_lower_split(d->st, n, cnt, d->Dbuff, nn, d->Tp + (size_t) b0 * K1p, K1p, K1p * r,
-                              d->Tdgp + (size_t) b0 * ldn, ldn, (long) ldn * r);
+        // lower triangle of T_p = (L^-T)^T V only (symmetric; column i of the left operand
+        // L^-T is zero below row i, tri 2), written as packed planes of its strictly lower
+        // part and of its diagonal, the two operands of the Gram products below
+        hdk::gemm_bf16p_packed(d->st, n, ldn, 1.0, d->LiTp, ldn, nnp, 0, d->Vp, ldn, bps, nnp,
+                               d->Tp + (size_t) b0 * K1p, K1p, K1p * r, d->Tdgp + (size_t) b0 * ldn, ldn,
+                               (long) ldn * r, cnt, 2);
     }
     const long kc = d->chunkP;
     const int full = (int) (K1p / kc);
@@ -1034,7 +1038,7 @@ hdsdp_retcode hdsdp_dev_load(hdsdp_dev *d, const hdsdp_coeff_view *cv) {
     if (c.nR1 > 0 || c.cDSR1) DCALL(dalloc(d, &d->H1, (size_t) n * (c.nR1 + 1)));
     if (d->r1Gram && !d->r1Identity) DCALL(dalloc(d, &d->Kr, (size_t) c.nR1 * c.nR1));
     d->LiTf = d->Ff = d->Gf = d->Tf = d->Tdgf = d->Dbuff = d->Vf = nullptr;
-    d->LiTp = d->Dp = d->Vp = d->Tp = d->Tdgp = nullptr;
+    d->LiTp = d->Dp = d->Vp = d->Tp = d->Tdgp = d->Dall = nullptr;
     d->K1p = d->chunkP = 0;
     d->Wm = d->Lim = d->mDiag0 = d->mDiagCur = d->xCoef = nullptr;
     // bf16-split Gram blocks need the matrix-free FP64 product in the refinement.  For dense
@@ -1093,11 +1097,27 @@ hdsdp_retcode hdsdp_dev_load(hdsdp_dev *d, const hdsdp_coeff_view *cv) {
             // Gpart holds the parts of either build (the FP64 rebuild uses gramChunk)
             d->gramParts = std::max<int>(d->gramParts, (int) ((d->K1p + d->chunkP - 1) / d->chunkP));
             DCALL(dalloc_zero(d, &d->LiTp, 3 * nnp));
-            DCALL(dalloc_zero(d, &d->Dp, 3 * nnp * d->gramBatch));
+            // The constraints never change: their planes are split once and kept, 6 r n^2
+            // bytes (12 GB for n = 1000, r = 2000), while that stays under
+            // HDSDP_PLANES_RESIDENT_GB (default 96); otherwise each build splits a batch.
+            double residentGb = 96.0;
+            if (const char *rg = getenv("HDSDP_PLANES_RESIDENT_GB")) residentGb = atof(rg);
+            const size_t allCount = 3 * nnp * (size_t) r;
+            if ((double) allCount * sizeof(__bf16) <= residentGb * 1e9) {
+                void *q = nullptr;
+                if (hipMalloc(&q, allCount * sizeof(__bf16)) == hipSuccess) {
+                    d->allocs.push_back(q);
+                    d->Dall = (__bf16 *) q;
+                    HIP_OK(hipMemsetAsync(d->Dall, 0, allCount * sizeof(__bf16), d->st));
+                    hdk::unpack_dense_split(d->st, d->c, 0, r, d->Dall, (int) ldn, (long) nnp * r, (long) nnp);
+                } else {
+                    (void) hipGetLastError();  // the batch path below needs far less memory
+                }
+            }
+            if (!d->Dall) DCALL(dalloc_zero(d, &d->Dp, 3 * nnp * d->gramBatch));
             DCALL(dalloc_zero(d, &d->Vp, 3 * nnp * d->gramBatch));
             DCALL(dalloc_zero(d, &d->Tp, 3 * (size_t) d->K1p * r));
             DCALL(dalloc_zero(d, &d->Tdgp, 3 * ldn * r));
-            DCALL(dalloc(d, &d->Dbuff, nn * d->gramBatch));
         } else if (mixedGram) {
             DCALL(dalloc(d, &d->Tf, K1 * r));
             DCALL(dalloc(d, &d->Tdgf, (size_t) n * r));
@@ -1319,6 +1339,7 @@ hdsdp_retcode hdsdp_dev_schur(hdsdp_dev *d, int typeKKT, double Rd, double *asin
             HIP_OK(hipMemcpyAsync(d->Lim, d->Li[0], sizeof(double) * (size_t) n * n, hipMemcpyDeviceToDevice, d->st));
         }
         if (d->r1Gram || d->dnGram) prof_begin(d, HDSDP_PROF_GRAM);
+        if (mixed) prof_begin(d, HDSDP_PROF_GRAM16);
         if (d->r1Gram) {
             if (mixed) r1_gram_mixed(d);
             else if (d->r1Identity) hdk::r1_schur(d->st, d->c, d->H1, d->M, m, d->M, m);
@@ -1328,6 +1349,7 @@ hdsdp_retcode hdsdp_dev_schur(hdsdp_dev *d, int typeKKT, double Rd, double *asin
             if (mixed) dense_gram_mixed(d);
             else dense_gram(d, d->Li[0]);
         }
+        if (mixed) DCALL(prof_end(d, HDSDP_PROF_GRAM16));
         if (d->r1Gram || d->dnGram) DCALL(prof_end(d, HDSDP_PROF_GRAM));
         for (int h : d->heavy) {
             // B_h = W A_h W, then M(h, k) = <A_k, B_h>

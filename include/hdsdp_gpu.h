@@ -223,6 +223,8 @@ hdsdp_retcode hdsdp_dev_stats( hdsdp_dev *dev, double *stats, int nStats );
  *   HDSDP_PROF_PRIMAL    primal recovery
  *   HDSDP_PROF_GRAM      the Gram products of a Schur build (dense: T_p = L^-1 A_p L^-T
  *                        and <T_p, T_q>; rank-one: W F and F^T W F), FP64 or bf16 split
+ *   HDSDP_PROF_GRAM16    the same products of the builds made from bf16 split products
+ *                        only (mixed precision; the builds made in FP64 are left out)
  * Profiling is off by default: enabling it adds one event synchronisation per
  * timed region.  No counterpart in the reference (measurement only). */
 #define HDSDP_PROF_SCHUR   0
@@ -234,7 +236,8 @@ hdsdp_retcode hdsdp_dev_stats( hdsdp_dev *dev, double *stats, int nStats );
 #define HDSDP_PROF_BUFFER  6
 #define HDSDP_PROF_PRIMAL  7
 #define HDSDP_PROF_GRAM    8
-#define HDSDP_PROF_SLOTS   9
+#define HDSDP_PROF_GRAM16  9
+#define HDSDP_PROF_SLOTS   10
 hdsdp_retcode hdsdp_dev_set_profiling( hdsdp_dev *dev, int enable );
 int hdsdp_dev_get_profile( hdsdp_dev *dev, double *ms, double *counts, int nSlots );
 

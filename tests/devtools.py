@@ -366,9 +366,12 @@ def check_parity(fix, res, label=""):
     return reps
 
 
-def adjoint_fd_check(lib, inst, n_dirs, seed=0, h=1e-6, relopt=1e-8, **solve_opts):
-    """hdsdp_amd.adjoint.fd_check, returned as (relative errors, mu, dL/db, dL/dC, y_mu)."""
+def adjoint_fd_check(lib, inst, n_dirs, seed=0, h=1e-6, relopt=1e-8, noise=False, **solve_opts):
+    """hdsdp_amd.adjoint.fd_check, returned as (relative errors, mu, dL/db, dL/dC, y_mu)
+    and, with noise, the finite-difference error estimates per direction as well."""
     from hdsdp_amd.adjoint import fd_check
 
-    r = fd_check(lib, inst, n_dirs, seed=seed, h=h, relopt=relopt, **solve_opts)
+    r = fd_check(lib, inst, n_dirs, seed=seed, h=h, relopt=relopt, noise=noise, **solve_opts)
+    if noise:
+        return r["errs"], r["mu"], r["dLdb"], r["dLdC"], r["y"], r["fd_noise"]
     return r["errs"], r["mu"], r["dLdb"], r["dLdC"], r["y"]

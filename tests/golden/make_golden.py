@@ -42,7 +42,8 @@ RUNNER_DENSE_FMA = os.path.join(ROOT, "oracle", "_ref", "dense_fma", "hdsdp_ref_
 #             PARDISO (interface/hdsdp_conic_sdp.c:65-97); the dense-dual build
 #             (oracle/_ref/dense) is run as well and its result kept beside the default one
 #   "heavy":  the spread runs use the default thread count only (one thread takes minutes)
-#   "medium": the spread runs use both thread counts over the first three MKL paths
+#   "medium": the spread runs use both thread counts over every MKL path (as the light
+#             cases), listed for instances whose runs take about a minute
 #   "dense_primary": the primary result comes from the dense-dual build (the default build
 #             stops with a segmentation fault on this instance: its primal method symmetrises
 #             the dual step as a dense n x n array while PARDISO holds it sparse,
@@ -135,7 +136,7 @@ def spread_runs(inst, opts, flags, primary):
     elif "sparse" in flags:
         builds += [("dense", RUNNER_DENSE), ("dense_fma", RUNNER_DENSE_FMA)]
     threads = [None] if "heavy" in flags else [None, 1]
-    paths = CBWR_PATHS[:3] if ("heavy" in flags or "medium" in flags) else CBWR_PATHS
+    paths = CBWR_PATHS[:3] if "heavy" in flags else CBWR_PATHS
     runs, results = [], []
     for bname, runner in builds:
         for th in threads:
