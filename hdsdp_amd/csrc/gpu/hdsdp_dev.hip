@@ -98,7 +98,8 @@ struct hdsdp_dev {
     double *jac, *cr, *crn, *cp, *cz, *cMp, *cx, *crhs, *ctmp;
     double *precS;     // Jacobi scaling of the FP32 preconditioner
     double precShift;  // diagonal shift the FP32 factorisation needed
-    double refineFactor;  // mixed: the exact-operator solves stop at refineFactor * tol
+    double mixedFloor;  // mixed: refinement against the exact operator stops at this fraction of ||b||
+    double *cb, *cxa;   // mixed refinement: right-hand side and accumulated solution
     int precKind;
     int useJacobi, paramUseJacobi, maxIter, nRestart;
     double absTol, relTol;
@@ -463,23 +464,13 @@ void schur_apply_exact(hdsdp_dev *d, const double *x, double *y) {
     hdk::diag_delta_axpy(d->st, m, d->M, m, d->mDiag0, x, y);
 }
 
-// conjGradSolve (linalg/hdsdp_linsolver.c:855-997) on crhs; result in cx.
-// Returns 0 on success, 1 if the solution is left unwritten (tiny rhs), -1 on failure.
-int cg_solve(hdsdp_dev *d) {
+int cg_solve(hdsdp_dev *d);
+
+// conjGradSolve (linalg/hdsdp_linsolver.c:855-997) on crhs with the M on the device;
+// result in cx.  Returns 0 on success, 1 if the solution is left unwritten (tiny rhs),
+// -1 on failure.
+int cg_solve_inner(hdsdp_dev *d) {
     const int m = d->m;
-    // While M holds bf16-split Gram blocks (FP32-accurate) the control flow is that of
-    // the FP64 path: Jacobi-preconditioned CG on that M while the reference would use
-    // Jacobi (its solves stop at a tolerance far above the error of M), and once it has
-    // switched to the factor preconditioner, CG on the exact FP64 operator (applied
-    // matrix-free) with the Cholesky factor of that M as preconditioner, to 1e-4 tol
-    // (near the accuracy of an FP64 factor solve); a solve that cannot get below tol
-    // fails, and M is then rebuilt in FP64.
-    d->useJacobi = d->paramUseJacobi;
-    const bool exactOp = d->mGramPending != 0 && !d->useJacobi;
-    auto op = [&](const double *x, double *y) {
-        if (exactOp) schur_apply_exact(d, x, y);
-        else symv_M(d, x, y);
-    };
     const int restart = std::max(d->nRestart, 20);
     hdk::cg_start(d->st, m, d->crhs, d->cr, d->cx, d->sc_d);
     if (read_scalars(d) != HDSDP_RETCODE_OK) return -1;
@@ -487,9 +478,8 @@ int cg_solve(hdsdp_dev *d) {
     double tol = std::min(d->absTol, bnorm * d->relTol);
     tol = std::max(tol, 0.1 * d->absTol);
     if (rnorm < tol) return 1;
-    const double target = exactOp ? d->refineFactor * tol : tol;
     apply_precond(d, d->cr, d->cz, d->cp);
-    op(d->cp, d->cMp);
+    symv_M(d, d->cp, d->cMp);
     int iter = 0, status = 0, refine = 0;
     double prevTrue = 0.0, rtrue = -1.0;
     for (iter = 0; iter < d->maxIter; ++iter) {
@@ -505,7 +495,7 @@ int cg_solve(hdsdp_dev *d) {
         }
         apply_precond(d, d->crn, d->cz, nullptr);
         hdk::cg_step2(d->st, m, d->crn, d->cz, d->cp, d->cr, d->sc_d);
-        op(d->cp, d->cMp);
+        symv_M(d, d->cp, d->cMp);
         if (read_scalars(d) != HDSDP_RETCODE_OK) return -1;
         rnorm = d->hsc[hdk::CG_RNORM];
         if (rnorm != rnorm) return -1;
@@ -513,41 +503,35 @@ int cg_solve(hdsdp_dev *d) {
             status = 2;
             break;
         }
-        if (rnorm < target) {
+        if (rnorm < tol) {
             // With the FP32 preconditioner the recursively updated residual drifts
             // from b - M x: confirm convergence on the true FP64 residual and, if it
             // is not met, restart from x (iterative refinement).  Refinement stops at
-            // the target or once the true residual stops halving (the rounding level
-            // of an FP64 product with M).
+            // tol or once the true residual stops halving (the rounding level of an
+            // FP64 product with M).
             if (d->precision != HDSDP_PRECISION_MIXED || d->useJacobi) break;
-            if (refine >= 8) {
-                if (exactOp && rtrue >= tol) return -1;
-                break;
-            }
-            op(d->cx, d->ctmp);
+            if (refine >= 8) break;
+            symv_M(d, d->cx, d->ctmp);
             hdk::cg_true_residual(d->st, m, d->crhs, d->ctmp, d->cr, d->sc_d);
             if (read_scalars(d) != HDSDP_RETCODE_OK) return -1;
             rtrue = d->hsc[hdk::CG_TRUE];
             if (rtrue != rtrue) return -1;
-            if (rtrue < target) break;
-            if (refine > 0 && rtrue > 0.5 * prevTrue) {
-                if (exactOp && rtrue >= tol) return -1;
-                break;
-            }
+            if (rtrue < tol) break;
+            if (refine > 0 && rtrue > 0.5 * prevTrue) break;
             prevTrue = rtrue;
             ++refine;
             apply_precond(d, d->cr, d->cz, d->cp);
-            op(d->cp, d->cMp);
+            symv_M(d, d->cp, d->cMp);
         }
     }
     d->stats[5] = iter;
     static const int trace = getenv("HDSDP_TRACE_CG") != nullptr;
     if (trace)
         fprintf(stderr,
-                "[hdsdp_amd] cg m=%d precond=%s iters=%d refine=%d%s rnorm=%.3e rtrue=%.3e bnorm=%.3e tol=%.3e "
+                "[hdsdp_amd] cg m=%d precond=%s iters=%d refine=%d rnorm=%.3e rtrue=%.3e bnorm=%.3e tol=%.3e "
                 "status=%d\n",
                 m, d->useJacobi ? "jacobi" : (d->precKind == PREC_CHOL32 ? "chol-fp32" : "chol-fp64"), iter, refine,
-                exactOp ? " (exact op)" : "", rnorm, rtrue, bnorm, tol, status);
+                rnorm, rtrue, bnorm, tol, status);
     if (iter >= d->maxIter || status == 2) {
         if (d->useJacobi) {
             d->useJacobi = 0;
@@ -558,11 +542,62 @@ int cg_solve(hdsdp_dev *d) {
         if (d->precKind == PREC_CHOL32) {
             // the FP32 preconditioner was not good enough for this system: FP64 factor
             if (build_chol64(d) != HDSDP_RETCODE_OK) return -1;
-            return cg_solve(d);
+            return cg_solve_inner(d);
         }
         return -1;
     }
     return 0;
+}
+
+// While the Gram blocks of M hold bf16-split products (FP32-accurate) and the
+// factor preconditioner is in use, the system solved is the exact FP64 one: iterative
+// refinement x += M~^-1 (b - M x), the residual from the matrix-free FP64 product and
+// each correction from CG on the device M~ (products with M~, preconditioned by its
+// factor).  Converged once ||b - M x|| < max(tol, mixedFloor ||b||), the level an FP64
+// direct solve reaches (a relative 1e-15 or so of ||b|| once M is ill-conditioned);
+// fails when a step does not halve the residual before that, and M is then rebuilt in
+// FP64.
+int cg_solve_refined(hdsdp_dev *d) {
+    const int m = d->m;
+    HIP_OK(hipMemcpyAsync(d->cb, d->crhs, sizeof(double) * m, hipMemcpyDeviceToDevice, d->st));
+    hdk::axpy_norm(d->st, m, 0.0, d->cb, d->cb, d->sc_d, hdk::SC_AUX);
+    if (read_scalars(d) != HDSDP_RETCODE_OK) return -1;
+    const double bnorm = d->hsc[hdk::SC_AUX];
+    const double tol = std::max(std::min(d->absTol, bnorm * d->relTol), 0.1 * d->absTol);
+    const double accept = std::max(tol, d->mixedFloor * bnorm);
+    if (bnorm < tol) return 1;
+    hdk::fill(d->st, d->cxa, m, 0.0);
+    static const int trace = getenv("HDSDP_TRACE_CG") != nullptr;
+    double prev = HUGE_VAL, rtrue = bnorm;
+    int rc = -1, step = 0;
+    for (step = 0; step < 8; ++step) {
+        const int inner = cg_solve_inner(d);
+        if (inner < 0) break;
+        if (inner == 0) hdk::axpy(d->st, m, 1.0, d->cx, d->cxa);
+        schur_apply_exact(d, d->cxa, d->ctmp);
+        hdk::cg_true_residual(d->st, m, d->cb, d->ctmp, d->crhs, d->sc_d);
+        if (read_scalars(d) != HDSDP_RETCODE_OK) break;
+        rtrue = d->hsc[hdk::CG_TRUE];
+        if (trace)
+            fprintf(stderr, "[hdsdp_amd] refine m=%d step=%d rtrue=%.3e (%.2e of bnorm) accept=%.3e\n", m, step, rtrue,
+                    rtrue / bnorm, accept);
+        if (rtrue != rtrue) break;
+        if (rtrue < accept) {
+            rc = 0;
+            break;
+        }
+        if (inner == 1 || rtrue > 0.5 * prev) break;
+        prev = rtrue;
+    }
+    HIP_OK(hipMemcpyAsync(d->cx, d->cxa, sizeof(double) * m, hipMemcpyDeviceToDevice, d->st));
+    HIP_OK(hipMemcpyAsync(d->crhs, d->cb, sizeof(double) * m, hipMemcpyDeviceToDevice, d->st));
+    return rc;
+}
+
+int cg_solve(hdsdp_dev *d) {
+    d->useJacobi = d->paramUseJacobi;
+    if (d->mGramPending && !d->useJacobi) return cg_solve_refined(d);
+    return cg_solve_inner(d);
 }
 
 // M(p, q) = <T_p, T_q> over the dense constraints with T_p = L^-1 A_p L^-T (so that
@@ -807,12 +842,14 @@ hdsdp_retcode hdsdp_dev_create(hdsdp_dev **pdev, int nRow, int nCol, int precisi
     DCALL(dalloc(d, &d->cx, m));
     DCALL(dalloc(d, &d->crhs, m));
     DCALL(dalloc(d, &d->ctmp, m));
+    DCALL(dalloc(d, &d->cb, m));
+    DCALL(dalloc(d, &d->cxa, m));
     // CG defaults (linalg/hdsdp_linsolver.c:718-728)
     d->absTol = 1e-06;
     d->relTol = 1e-06;
     d->useJacobi = d->paramUseJacobi = 1;
-    d->refineFactor = 1e-4;
-    if (const char *rf = getenv("HDSDP_MIXED_REFINE")) d->refineFactor = atof(rf);
+    d->mixedFloor = 1e-13;
+    if (const char *mf = getenv("HDSDP_MIXED_FLOOR")) d->mixedFloor = atof(mf);
     d->mVer = 0;
     d->cholVer = -1;
     d->maxIter = std::max(50, nRow / 20);
