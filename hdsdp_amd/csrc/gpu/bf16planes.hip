@@ -54,10 +54,117 @@ __device__ inline void split3(float x, __bf16 &a, __bf16 &b, __bf16 &c) {
     c = (__bf16) (r1 - (float) b);
 }
 
+// Writes the 128 x 128 output tile staged in LDS (Ct[column][row], FP32, already scaled
+// by alpha) with NT threads.  OUT 3: the packed strictly-lower part and the diagonal as
+// three bf16 planes, one row per thread, so a wave's stores of one column are
+// consecutive packed entries.  OUT 2 / OUT 0: three bf16 planes / FP32, each thread
+// writing four consecutive rows of one column, so a wave covers whole 128-row column
+// segments.  C, Dg point at this batch entry.
+template <int OUT, bool LOWER, int NT>
+__device__ inline void store_staged(const float (*Ct)[PT + CPAD], int tid, int i0, int j0, int M, int N, void *Cv,
+                                    int ldc, long psC, __bf16 *dg, long psDg) {
+    if constexpr (OUT == 3) {
+        __bf16 *off = static_cast<__bf16 *>(Cv);
+        const int il = tid & (PT - 1), jq = tid / PT, i = i0 + il;
+        for (int jl = jq; jl < PT; jl += NT / PT) {
+            const int j = j0 + jl;
+            if (j >= N || i >= M || i < j) continue;
+            __bf16 p0, p1, p2;
+            split3(Ct[jl][il], p0, p1, p2);
+            if (i > j) {
+                const long e = (long) j * M - (long) j * (j + 1) / 2 + (i - j - 1);
+                off[e] = p0;
+                off[e + psC] = p1;
+                off[e + 2 * psC] = p2;
+            } else {
+                dg[i] = p0;
+                dg[i + psDg] = p1;
+                dg[i + 2 * psDg] = p2;
+            }
+        }
+    } else {
+        const int il = (tid & 31) * 4, jt = tid >> 5;  // rows il .. il + 3, columns jt + (NT / 32) q
+        const int i = i0 + il;
+#pragma unroll
+        for (int q = 0; q < PT / (NT / 32); ++q) {
+            const int jl = jt + (NT / 32) * q, j = j0 + jl;
+            if (j >= N) continue;
+            const float4 v = *reinterpret_cast<const float4 *>(&Ct[jl][il]);
+            const float vv[4] = {v.x, v.y, v.z, v.w};
+            const bool full = i + 3 < M && (!LOWER || i >= j);
+            const long e = i + (long) j * ldc;
+            if constexpr (OUT == 2) {
+                __bf16 *C = static_cast<__bf16 *>(Cv);
+                __bf16 pl[3][4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) split3(vv[u], pl[0][u], pl[1][u], pl[2][u]);
+                if (full && (e & 3) == 0 && ((psC & 3) == 0)) {
+#pragma unroll
+                    for (int t = 0; t < 3; ++t) {
+                        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+                        *reinterpret_cast<bf16x4 *>(C + e + t * psC) = (bf16x4){pl[t][0], pl[t][1], pl[t][2], pl[t][3]};
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (i + u < M && (!LOWER || i + u >= j)) {
+                            C[e + u] = pl[0][u];
+                            C[e + u + psC] = pl[1][u];
+                            C[e + u + 2 * psC] = pl[2][u];
+                        }
+                }
+            } else {
+                float *C = static_cast<float *>(Cv);
+                if (full && (e & 3) == 0) {
+                    *reinterpret_cast<float4 *>(C + e) = v;
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (i + u < M && (!LOWER || i + u >= j)) C[e + u] = vv[u];
+                }
+            }
+        }
+    }
+}
+
+// Lower tile of an mt x mt tile grid for workgroup id (0 <= id < ntile = mt (mt + 1) / 2).
+// Workgroups go to the eight XCDs in turn (XCD id % 8), and XCD c is given a run of
+// consecutive tiles of the blocked order, which lists the lower tiles super-block by
+// super-block (4 x 4 tiles, row-major, each row-major inside): the tiles that one XCD
+// holds read few strips of the operands, so its own L2 serves their re-reads.
+__device__ inline void lower_tile_xcd(int id, int ntile, int mt, int &bx, int &by) {
+    const int c = id % 8, q = id / 8, base = ntile / 8, rem = ntile % 8;
+    const int t = c * base + min(c, rem) + q;
+    constexpr int G = 4;
+    const int nb = (mt + G - 1) / G;
+    int acc = 0;
+    for (int I = 0; I < nb; ++I)
+        for (int J = 0; J <= I; ++J) {
+            const int r0 = I * G, r1 = min(mt, r0 + G), c0 = J * G, c1 = min(mt, c0 + G);
+            const int w = r1 - r0, cnt = (I > J) ? w * (c1 - c0) : w * (w + 1) / 2;
+            if (t < acc + cnt) {
+                const int u = t - acc;
+                if (I > J) {
+                    bx = r0 + u / (c1 - c0);
+                    by = c0 + u % (c1 - c0);
+                } else {
+                    int x = 0;
+                    while ((x + 1) * (x + 2) / 2 <= u) ++x;
+                    bx = r0 + x;
+                    by = c0 + (u - x * (x + 1) / 2);
+                }
+                return;
+            }
+            acc += cnt;
+        }
+    bx = by = 0;  // not reached for id < ntile
+}
+
 // tri: 1 when Y(k, j) = 0 for k > j, 2 when X(k, i) = 0 for k > i (k range of a tile
 // ends at its last column or row).  With tri == 2 on a 2-D grid the tile row is rotated
 // by the other grid coordinates, so that every XCD gets every row (the work of a tile
-// grows with its row).  lin: blockIdx.x runs over the lower tiles only.
+// grows with its row).  lin: blockIdx.x runs over the lower tiles only, row by row (1)
+// or in the XCD-grouped order of lower_tile_xcd (2).
 // OUT: 0 = FP32 C, 1 = FP64 C, 2 = three bf16 planes of C (at C + t * psC, ld ldc),
 // 3 = three bf16 planes of a symmetric C (M = N = n, LOWER) packed: the strictly lower
 // part column by column (entry (i, j), i > j, at j n - j (j + 1) / 2 + i - j - 1 of C,
@@ -71,7 +178,9 @@ __global__ __launch_bounds__(PTHREADS) void gemm_bf16p_kernel(int M, int N, int 
                                                               int lin, int tri, __bf16 *Dg, long sDg, long psDg) {
     int bx = blockIdx.x, by = blockIdx.y;
     if (!lin && tri == 2) bx = (int) ((blockIdx.x + blockIdx.y + blockIdx.z) % gridDim.x);
-    if (lin) {
+    if (lin == 2) {
+        lower_tile_xcd(blockIdx.x, gridDim.x, (M + PT - 1) / PT, bx, by);
+    } else if (lin) {
         const int t = blockIdx.x;
         int x = (int) ((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
         while ((x + 1) * (x + 2) / 2 <= t) ++x;
@@ -185,117 +294,26 @@ __global__ __launch_bounds__(PTHREADS) void gemm_bf16p_kernel(int M, int N, int 
         __syncthreads();
     }
     // epilogue: lane holds rows i0 + wi + 16 a + 4 lg + r of column j0 + wj + 16 b + lr
-    if constexpr (OUT == 3) {
-        // staged through LDS, then written one row per lane: a wave's stores of one
-        // column are consecutive packed entries
-        float(*Ct)[PT + CPAD] = reinterpret_cast<float(*)[PT + CPAD]>(smem);
+    if constexpr (OUT == 3 || OUT == 2 || OUT == 0) {
+        if (OUT != 0 || beta == 0.0) {
+            float(*Ct)[PT + CPAD] = reinterpret_cast<float(*)[PT + CPAD]>(smem);
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+            for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int jl = wj + 16 * b + lr;
+                for (int b = 0; b < 4; ++b) {
+                    const int jl = wj + 16 * b + lr;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) Ct[jl][wi + 16 * a + 4 * lg + r] = (float) (alpha * acc64[a][b][r]);
-            }
-        __syncthreads();
-        __bf16 *off = static_cast<__bf16 *>(Cv) + (long) blockIdx.z * sC;
-        __bf16 *dg = Dg + (long) blockIdx.z * sDg;
-        const int il = tid & (PT - 1), jq = tid / PT, i = i0 + il;
-        for (int jl = jq; jl < PT; jl += PTHREADS / PT) {
-            const int j = j0 + jl;
-            if (j >= N || i >= M || i < j) continue;
-            __bf16 p0, p1, p2;
-            split3(Ct[jl][il], p0, p1, p2);
-            if (i > j) {
-                const long e = (long) j * M - (long) j * (j + 1) / 2 + (i - j - 1);
-                off[e] = p0;
-                off[e + psC] = p1;
-                off[e + 2 * psC] = p2;
-            } else {
-                dg[i] = p0;
-                dg[i + psDg] = p1;
-                dg[i + 2 * psDg] = p2;
-            }
+                    for (int r = 0; r < 4; ++r) Ct[jl][wi + 16 * a + 4 * lg + r] = (float) (alpha * acc64[a][b][r]);
+                }
+            __syncthreads();
+            const size_t esz = (OUT == 0) ? sizeof(float) : sizeof(__bf16);
+            store_staged<OUT, LOWER, PTHREADS>(Ct, tid, i0, j0, M, N,
+                                               static_cast<char *>(Cv) + (long) blockIdx.z * sC * (long) esz, ldc, psC,
+                                               Dg + (long) blockIdx.z * sDg, psDg);
+            return;
         }
-        return;
     }
-    if (OUT == 2 || (OUT == 0 && beta == 0.0)) {
-        // Staged through LDS (Ct[column][row], FP32), then written column by column: a
-        // thread writes four consecutive rows of one column, so a wave's stores cover
-        // whole 128-row column segments instead of one element per column.
-        float(*Ct)[PT + CPAD] = reinterpret_cast<float(*)[PT + CPAD]>(smem);
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int jl = wj + 16 * b + lr;
-#pragma unroll
-                for (int r = 0; r < 4; ++r) Ct[jl][wi + 16 * a + 4 * lg + r] = (float) (alpha * acc64[a][b][r]);
-            }
-        __syncthreads();
-        const int il = (tid & 31) * 4, jt = tid >> 5;  // rows il .. il + 3, columns jt + 16 q
-        const int i = i0 + il;
-#pragma unroll
-        for (int q = 0; q < PT / 16; ++q) {
-            const int jl = jt + 16 * q, j = j0 + jl;
-            if (j >= N) continue;
-            const float4 v = *reinterpret_cast<const float4 *>(&Ct[jl][il]);
-            const float vv[4] = {v.x, v.y, v.z, v.w};
-            const bool full = i + 3 < M && (!LOWER || i >= j);
-            if constexpr (OUT == 2) {
-                __bf16 *C = static_cast<__bf16 *>(Cv) + (long) blockIdx.z * sC;
-                const long e = i + (long) j * ldc;
-                __bf16 p[3][4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) split3(vv[u], p[0][u], p[1][u], p[2][u]);
-                if (full && (e & 3) == 0 && ((psC & 3) == 0)) {
-#pragma unroll
-                    for (int t = 0; t < 3; ++t) {
-                        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-                        *reinterpret_cast<bf16x4 *>(C + e + t * psC) = (bf16x4){p[t][0], p[t][1], p[t][2], p[t][3]};
-                    }
-                } else {
-#pragma unroll
-                    for (int u = 0; u < 4; ++u)
-                        if (i + u < M && (!LOWER || i + u >= j)) {
-                            C[e + u] = p[0][u];
-                            C[e + u + psC] = p[1][u];
-                            C[e + u + 2 * psC] = p[2][u];
-                        }
-                }
-            } else {
-                float *C = static_cast<float *>(Cv) + (long) blockIdx.z * sC;
-                const long e = i + (long) j * ldc;
-                if (full && (e & 3) == 0) {
-                    *reinterpret_cast<float4 *>(C + e) = v;
-                } else {
-#pragma unroll
-                    for (int u = 0; u < 4; ++u)
-                        if (i + u < M && (!LOWER || i + u >= j)) C[e + u] = vv[u];
-                }
-            }
-        }
-        return;
-    }
-    if constexpr (OUT == 2) {
-        __bf16 *C = static_cast<__bf16 *>(Cv) + (long) blockIdx.z * sC;
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int i = i0 + wi + 16 * a + 4 * lg + r, j = j0 + wj + 16 * b + lr;
-                    if (i < M && j < N && (!LOWER || i >= j)) {
-                        __bf16 p0, p1, p2;
-                        split3((float) (alpha * acc64[a][b][r]), p0, p1, p2);
-                        const long e = i + (long) j * ldc;
-                        C[e] = p0;
-                        C[e + psC] = p1;
-                        C[e + 2 * psC] = p2;
-                    }
-                }
-    } else {
+    if constexpr (OUT == 0 || OUT == 1) {
         typedef typename std::conditional<OUT == 1, double, float>::type Tout;
         Tout *C = static_cast<Tout *>(Cv) + (long) blockIdx.z * sC;
         Tout cin[2][4][4];
@@ -329,9 +347,10 @@ void launch_p(hipStream_t s, bool lower, int M, int N, int K, double alpha, cons
     const int mt = (M + PT - 1) / PT, nt = (N + PT - 1) / PT;
     const int lin = (lower && mt == nt) ? 1 : 0;
     dim3 grid(lin ? mt * (mt + 1) / 2 : mt, lin ? 1 : nt, batch);
+    // lower tiles of a square grid in the XCD-grouped order (lin 2)
     if (lower)
         gemm_bf16p_kernel<OUT, true><<<grid, PTHREADS, 0, s>>>(M, N, K, alpha, X, ldx, psX, sX, Y, ldy, psY, sY, beta,
-                                                               C, ldc, psC, sC, lin, tri, Dg, sDg, psDg);
+                                                               C, ldc, psC, sC, lin ? 2 : 0, tri, Dg, sDg, psDg);
     else if constexpr (OUT != 3)
         gemm_bf16p_kernel<OUT, false><<<grid, PTHREADS, 0, s>>>(M, N, K, alpha, X, ldx, psX, sX, Y, ldy, psY, sY,
                                                                 beta, C, ldc, psC, sC, 0, tri, Dg, sDg, psDg);
