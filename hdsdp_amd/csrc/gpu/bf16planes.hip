@@ -21,8 +21,10 @@
 // from the true length up to ld are zero, so products run over whole 8-element groups.
 //
 // Tiles: 128 x 128 per workgroup of eight waves (4 x 2), each wave 32 x 64 (2 x 4 MFMA
-// 16 x 16 blocks); k slabs of 32 double-buffered in LDS (rows padded to 40 bf16 so that a
-// lane's 8 consecutive k are one aligned 16-byte read).  MFMA operand layout (gfx950):
+// 16 x 16 blocks); k slabs of 32 double-buffered in LDS (64-byte rows, the four 16-byte
+// k groups of a row swizzled by the row so that both the stores and the fragment reads
+// are free of bank conflicts; a lane's 8 consecutive k are one aligned 16-byte read).
+// MFMA operand layout (gfx950):
 // A(i, k) for i = lane % 16 and k = 8 (lane / 16) + t in element t; B(k, j) likewise with
 // j = lane % 16; D(i, j) at lane j + 16 (i / 4), element i % 4.
 #include "dev_common.h"
@@ -40,7 +42,7 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int PT = 128;     // output tile
 constexpr int PK = 32;      // k slab = one MFMA k
-constexpr int PPAD = 8;     // row padding (bf16)
+constexpr int PPAD = 0;     // row padding (bf16): none, the 16-byte k groups are swizzled instead
 constexpr int PFLUSH = 16;  // slabs between FP64 accumulations (k = 512)
 constexpr int PTHREADS = 512;
 constexpr int PSLAB = 2 * 3 * PT * (PK + PPAD);  // bf16 elements of one operand's double buffer
@@ -203,6 +205,11 @@ __global__ __launch_bounds__(PTHREADS) void gemm_bf16p_kernel(int M, int N, int 
     // loader: thread -> row tid / 4 of the tile, k group (tid & 3) * 8, one 16-byte
     // vector per plane and operand
     const int lrow = tid >> 2, lk = (tid & 3) * 8;
+    // LDS rows are 64 bytes with the 16-byte k group g of row r stored at g ^ ((r / 4) % 4):
+    // a quad of lanes stores one row and sixteen lanes four consecutive rows (64 banks),
+    // and the sixteen rows of a fragment read (rows 16 q + lane % 16) meet four distinct
+    // groups in each bank quarter, so neither side conflicts
+    const int lks = (((tid & 3) ^ ((lrow >> 2) & 3)) * 8), lgs = ((lg ^ ((lr >> 2) & 3)) * 8);
     const bool xrow = i0 + lrow < M, yrow = j0 + lrow < N;
     const __bf16 *xp = X + (long) (i0 + lrow) * ldx + lk;
     const __bf16 *yp = Y + (long) (j0 + lrow) * ldy + lk;
@@ -233,8 +240,8 @@ __global__ __launch_bounds__(PTHREADS) void gemm_bf16p_kernel(int M, int N, int 
     auto stash = [&](int buf) {
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
-            *reinterpret_cast<u32x4 *>(&As[buf][t][lrow][lk]) = vx[t];
-            *reinterpret_cast<u32x4 *>(&Bs[buf][t][lrow][lk]) = vy[t];
+            *reinterpret_cast<u32x4 *>(&As[buf][t][lrow][lks]) = vx[t];
+            *reinterpret_cast<u32x4 *>(&Bs[buf][t][lrow][lks]) = vy[t];
         }
     };
 
@@ -251,13 +258,13 @@ __global__ __launch_bounds__(PTHREADS) void gemm_bf16p_kernel(int M, int N, int 
         for (int t = 0; t < 3; ++t)
 #pragma unroll
             for (int b = 0; b < 4; ++b)
-                bfr[t][b] = *reinterpret_cast<const bf16x8 *>(&Bs[cur][t][wj + 16 * b + lr][8 * lg]);
+                bfr[t][b] = *reinterpret_cast<const bf16x8 *>(&Bs[cur][t][wj + 16 * b + lr][lgs]);
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
             bf16x8 afr[3];
 #pragma unroll
             for (int t = 0; t < 3; ++t)
-                afr[t] = *reinterpret_cast<const bf16x8 *>(&As[cur][t][wi + 16 * a + lr][8 * lg]);
+                afr[t] = *reinterpret_cast<const bf16x8 *>(&As[cur][t][wi + 16 * a + lr][lgs]);
             // term-major over the four column blocks, so consecutive MFMAs accumulate into
             // different registers; correction terms smallest first
 #pragma unroll
