@@ -163,10 +163,11 @@ __device__ inline void lower_tile_xcd(int id, int ntile, int mt, int &bx, int &b
 }
 
 // tri: 1 when Y(k, j) = 0 for k > j, 2 when X(k, i) = 0 for k > i (k range of a tile
-// ends at its last column or row).  With tri == 2 on a 2-D grid the tile row is rotated
-// by the other grid coordinates, so that every XCD gets every row (the work of a tile
-// grows with its row).  lin: blockIdx.x runs over the lower tiles only, row by row (1)
-// or in the XCD-grouped order of lower_tile_xcd (2).
+// ends at its last column or row), 3 and 4 as described in bf16planes.h.  With tri 2 or 3
+// on a 2-D grid the tile row is rotated by the other grid coordinates, so that every XCD
+// gets every row (the work of a tile depends on its row).  lin: blockIdx.x runs over the
+// lower tiles only, row by row (1) or in the XCD-grouped order of lower_tile_xcd (2); with
+// 4 added, workgroups are first reordered batch-major by XCD (below).
 // OUT: 0 = FP32 C, 1 = FP64 C, 2 = three bf16 planes of C (at C + t * psC, ld ldc),
 // 3 = three bf16 planes of a symmetric C (M = N = n, LOWER) packed: the strictly lower
 // part column by column (entry (i, j), i > j, at j n - j (j + 1) / 2 + i - j - 1 of C,
@@ -177,13 +178,31 @@ __global__ __launch_bounds__(PTHREADS) void gemm_bf16p_kernel(int M, int N, int 
                                                               const __bf16 *X, int ldx, long psX, long sX,
                                                               const __bf16 *Y, int ldy, long psY, long sY,
                                                               double beta, void *Cv, int ldc, long psC, long sC,
-                                                              int lin, int tri, __bf16 *Dg, long sDg, long psDg) {
-    int bx = blockIdx.x, by = blockIdx.y;
-    if (!lin && tri == 2) bx = (int) ((blockIdx.x + blockIdx.y + blockIdx.z) % gridDim.x);
+                                                              int lin, int tri, __bf16 *Dg, long sDg, long psDg,
+                                                              const __bf16 *X2, long psX2, long sX2,
+                                                              const __bf16 *Y2, long psY2, long sY2) {
+    // lin & 4: batch-major XCD order.  Workgroups go to the eight XCDs in turn (XCD g % 8
+    // for the linear workgroup index g), and XCD c is given a run of consecutive
+    // (batch entry, tile) pairs in batch-major order: whole batch entries per XCD, which
+    // keeps each entry's operands in one L2 and gives every XCD the same work.
+    int gx = blockIdx.x, gy = blockIdx.y, bz = blockIdx.z;
+    const bool batchOrder = (lin & 4) != 0;
+    if (batchOrder) {
+        const long tiles = (long) gridDim.x * gridDim.y, total = tiles * gridDim.z;
+        const long g = gx + (long) gridDim.x * (gy + (long) gridDim.y * bz);
+        const long c = g % 8, q = g / 8, base = total / 8, rem = total % 8;
+        const long t = c * base + (c < rem ? c : rem) + q;
+        bz = (int) (t / tiles);
+        gy = (int) ((t % tiles) / gridDim.x);
+        gx = (int) (t % gridDim.x);
+        lin &= 3;
+    }
+    int bx = gx, by = gy;
+    if (!lin && !batchOrder && (tri == 2 || tri == 3)) bx = (int) ((gx + gy + bz) % gridDim.x);
     if (lin == 2) {
-        lower_tile_xcd(blockIdx.x, gridDim.x, (M + PT - 1) / PT, bx, by);
+        lower_tile_xcd(gx, gridDim.x, (M + PT - 1) / PT, bx, by);
     } else if (lin) {
-        const int t = blockIdx.x;
+        const int t = gx;
         int x = (int) ((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
         while ((x + 1) * (x + 2) / 2 <= t) ++x;
         while (x * (x + 1) / 2 > t) --x;
@@ -192,9 +211,23 @@ __global__ __launch_bounds__(PTHREADS) void gemm_bf16p_kernel(int M, int N, int 
     }
     const int i0 = bx * PT, j0 = by * PT;
     if (LOWER && i0 + PT - 1 < j0) return;
-    const int Ke = (tri == 1) ? min(K, j0 + PT) : (tri == 2) ? min(K, i0 + PT) : K;
-    X += (long) blockIdx.z * sX;
-    Y += (long) blockIdx.z * sY;
+    // k range [kb, Ke) of the tile (Ke <= K, the padded length)
+    int kb = 0, Ke = (tri == 1) ? min(K, j0 + PT) : (tri == 2) ? min(K, i0 + PT) : K;
+    if (tri == 3) {
+        if (i0 > j0 + PT - 1) return;  // the product is upper triangular: nothing below
+        kb = i0;
+        Ke = min(K, j0 + PT);
+    } else if (tri == 4) {
+        Ke = min(K, min(i0, j0) + PT);
+    }
+    const int nslab1 = (Ke - kb + PK - 1) / PK;
+    const int nslab = X2 ? 2 * nslab1 : nslab1;
+    X += (long) bz * sX;
+    Y += (long) bz * sY;
+    if (X2) {
+        X2 += (long) bz * sX2;
+        Y2 += (long) bz * sY2;
+    }
     // planes [buffer][term][row][k]; the same memory stages the output tile at the end
     __shared__ __attribute__((aligned(16))) __bf16 smem[2 * PSLAB];
     auto As = reinterpret_cast<__bf16(*)[3][PT][PK + PPAD]>(smem);
@@ -213,6 +246,9 @@ __global__ __launch_bounds__(PTHREADS) void gemm_bf16p_kernel(int M, int N, int 
     const bool xrow = i0 + lrow < M, yrow = j0 + lrow < N;
     const __bf16 *xp = X + (long) (i0 + lrow) * ldx + lk;
     const __bf16 *yp = Y + (long) (j0 + lrow) * ldy + lk;
+    // the second pair (same layout) supplies the slabs from nslab1 on
+    const __bf16 *xp2 = X2 ? X2 + (long) (i0 + lrow) * ldx + lk : xp;
+    const __bf16 *yp2 = X2 ? Y2 + (long) (j0 + lrow) * ldy + lk : yp;
 
     f32x4 acc[2][4], cor[2][4];
     double acc64[2][4][4];
@@ -227,14 +263,41 @@ __global__ __launch_bounds__(PTHREADS) void gemm_bf16p_kernel(int M, int N, int 
         }
 
     u32x4 vx[3], vy[3];
-    auto fetch = [&](int k0) {
+    auto fetch = [&](int sl) {
+        const bool seg2 = sl >= nslab1;
+        const int k0 = kb + (seg2 ? sl - nslab1 : sl) * PK;
+        const __bf16 *xs = seg2 ? xp2 : xp, *ys = seg2 ? yp2 : yp;
+        const long pxs = seg2 ? psX2 : psX, pys = seg2 ? psY2 : psY;
         // k0 + lk .. k0 + lk + 7 lies inside [0, ld) whenever k0 + lk < Ke (Ke <= K, the
         // padded length, a multiple of 8)
         const bool kin = k0 + lk < Ke;
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
-            vx[t] = (xrow && kin) ? *reinterpret_cast<const u32x4 *>(xp + t * psX + k0) : (u32x4){0u, 0u, 0u, 0u};
-            vy[t] = (yrow && kin) ? *reinterpret_cast<const u32x4 *>(yp + t * psY + k0) : (u32x4){0u, 0u, 0u, 0u};
+            vx[t] = (xrow && kin) ? *reinterpret_cast<const u32x4 *>(xs + t * pxs + k0) : (u32x4){0u, 0u, 0u, 0u};
+            vy[t] = (yrow && kin) ? *reinterpret_cast<const u32x4 *>(ys + t * pys + k0) : (u32x4){0u, 0u, 0u, 0u};
+        }
+        if (tri == 3 && k0 + lk <= i0 + lrow) {
+            // X holds a symmetric matrix A; the product takes its lower triangle with the
+            // diagonal halved, X(k, i) -> 0 for k < i and X(i, i) / 2 (exact in each plane)
+            const int i = i0 + lrow;
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+#pragma unroll
+                for (int w = 0; w < 4; ++w) {
+                    unsigned u = vx[t][w];
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const int k = k0 + lk + 2 * w + h;
+                        const unsigned bits = (u >> (16 * h)) & 0xffffu;
+                        unsigned nb = bits;
+                        if (k < i) nb = 0u;
+                        else if (k == i) nb = (unsigned) __builtin_bit_cast(unsigned short,
+                                                   (__bf16) (0.5f * __builtin_bit_cast(float, bits << 16)));
+                        u = (u & ~(0xffffu << (16 * h))) | (nb << (16 * h));
+                    }
+                    vx[t][w] = u;
+                }
+            }
         }
     };
     auto stash = [&](int buf) {
@@ -248,11 +311,10 @@ __global__ __launch_bounds__(PTHREADS) void gemm_bf16p_kernel(int M, int N, int 
     fetch(0);
     stash(0);
     __syncthreads();
-    const int nslab = (Ke + PK - 1) / PK;
     for (int sl = 0; sl < nslab; ++sl) {
         const int cur = sl & 1;
         const bool more = sl + 1 < nslab;
-        if (more) fetch((sl + 1) * PK);
+        if (more) fetch(sl + 1);
         bf16x8 bfr[3][4];
 #pragma unroll
         for (int t = 0; t < 3; ++t)
@@ -315,14 +377,14 @@ __global__ __launch_bounds__(PTHREADS) void gemm_bf16p_kernel(int M, int N, int 
             __syncthreads();
             const size_t esz = (OUT == 0) ? sizeof(float) : sizeof(__bf16);
             store_staged<OUT, LOWER, PTHREADS>(Ct, tid, i0, j0, M, N,
-                                               static_cast<char *>(Cv) + (long) blockIdx.z * sC * (long) esz, ldc, psC,
-                                               Dg + (long) blockIdx.z * sDg, psDg);
+                                               static_cast<char *>(Cv) + (long) bz * sC * (long) esz, ldc, psC,
+                                               Dg + (long) bz * sDg, psDg);
             return;
         }
     }
     if constexpr (OUT == 0 || OUT == 1) {
         typedef typename std::conditional<OUT == 1, double, float>::type Tout;
-        Tout *C = static_cast<Tout *>(Cv) + (long) blockIdx.z * sC;
+        Tout *C = static_cast<Tout *>(Cv) + (long) bz * sC;
         Tout cin[2][4][4];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
@@ -350,24 +412,29 @@ __global__ __launch_bounds__(PTHREADS) void gemm_bf16p_kernel(int M, int N, int 
 template <int OUT>
 void launch_p(hipStream_t s, bool lower, int M, int N, int K, double alpha, const __bf16 *X, int ldx, long psX,
               long sX, const __bf16 *Y, int ldy, long psY, long sY, double beta, void *C, int ldc, long psC, long sC,
-              int batch, int tri, __bf16 *Dg = nullptr, long sDg = 0, long psDg = 0) {
+              int batch, int tri, __bf16 *Dg = nullptr, long sDg = 0, long psDg = 0, const __bf16 *X2 = nullptr,
+              long psX2 = 0, long sX2 = 0, const __bf16 *Y2 = nullptr, long psY2 = 0, long sY2 = 0) {
     const int mt = (M + PT - 1) / PT, nt = (N + PT - 1) / PT;
     const int lin = (lower && mt == nt) ? 1 : 0;
     dim3 grid(lin ? mt * (mt + 1) / 2 : mt, lin ? 1 : nt, batch);
-    // lower tiles of a square grid in the XCD-grouped order (lin 2)
+    // lower tiles of a square grid in the XCD-grouped order (lin 2); the batched products
+    // of the symmetric split (tri 3, 4), whose tiles differ in work, batch-major by XCD
+    const int order = (tri == 3 || tri == 4) && batch > 1 ? 4 : 0;
     if (lower)
         gemm_bf16p_kernel<OUT, true><<<grid, PTHREADS, 0, s>>>(M, N, K, alpha, X, ldx, psX, sX, Y, ldy, psY, sY, beta,
-                                                               C, ldc, psC, sC, lin ? 2 : 0, tri, Dg, sDg, psDg);
+                                                               C, ldc, psC, sC, (lin ? (order ? 1 : 2) : 0) | order,
+                                                               tri, Dg, sDg, psDg, X2, psX2, sX2, Y2, psY2, sY2);
     else if constexpr (OUT != 3)
         gemm_bf16p_kernel<OUT, false><<<grid, PTHREADS, 0, s>>>(M, N, K, alpha, X, ldx, psX, sX, Y, ldy, psY, sY,
-                                                                beta, C, ldc, psC, sC, 0, tri, Dg, sDg, psDg);
+                                                                beta, C, ldc, psC, sC, order, tri, Dg, sDg, psDg, X2, psX2, sX2, Y2, psY2, sY2);
 }
 
 }  // namespace
 
 void gemm_bf16p_packed(hipStream_t s, int n, int K, double alpha, const __bf16 *X, int ldx, long psX, long sX,
                        const __bf16 *Y, int ldy, long psY, long sY, __bf16 *off, long sOff, long psOff, __bf16 *dg,
-                       long sDg, long psDg, int batch, int tri) {
+                       long sDg, long psDg, int batch, int tri, const __bf16 *X2, long psX2, long sX2,
+                       const __bf16 *Y2, long psY2, long sY2) {
     if (n <= 0 || batch <= 0) return;
     const bool aligned = K % 8 == 0 && ldx % 8 == 0 && ldy % 8 == 0 && psX % 8 == 0 && psY % 8 == 0 &&
                          sX % 8 == 0 && sY % 8 == 0 && ((uintptr_t) X % 16) == 0 && ((uintptr_t) Y % 16) == 0 &&
@@ -377,8 +444,13 @@ void gemm_bf16p_packed(hipStream_t s, int n, int K, double alpha, const __bf16 *
                         "product skipped\n", K, ldx, ldy);
         return;
     }
+    if (X2 && (psX2 % 8 != 0 || psY2 % 8 != 0 || sX2 % 8 != 0 || sY2 % 8 != 0 || ((uintptr_t) X2 % 16) != 0 ||
+               ((uintptr_t) Y2 % 16) != 0)) {
+        fprintf(stderr, "[hdsdp_amd] gemm_bf16p_packed: second operand pair not 8-element aligned; product skipped\n");
+        return;
+    }
     launch_p<3>(s, true, n, n, K, alpha, X, ldx, psX, sX, Y, ldy, psY, sY, 0.0, off, 0, psOff, sOff, batch, tri, dg,
-                sDg, psDg);
+                sDg, psDg, X2, psX2, sX2, Y2, psY2, sY2);
 }
 
 namespace {

@@ -710,11 +710,12 @@ void dense_gram_mixed_inloop(hdsdp_dev *d) {
     hdk::gram_reduce(d->st, r, nOff + 1, d->Gpart, r, rr, d->c.denseCon, d->M, m);
 }
 
-// Mixed precision: the same products on the bf16 matrix cores (FP32-equivalent split
+// Mixed precision: the products on the bf16 matrix cores (FP32-equivalent split
 // products), written as X^T Y with both operands contiguous along k:
-//   V_p = A_p L^-T = (A_p)^T (L^-T),  T_p = L^-1 V_p = (L^-T)^T V_p,  M = T^T T.
+//   U_p^T = E_p^T L^-T,  T_p = U_p L^-T + (U_p L^-T)^T,  M = T^T T.
 // Every operand is held as three bf16 planes written by the kernel that produces it
-// (L^-T and A_p by split kernels, V_p by the first product's epilogue, T_p by the pack).
+// (L^-T and A_p by split kernels, U_p^T by the first product's epilogue, T_p by the
+// second's).
 void dense_gram_mixed(hdsdp_dev *d) {
     const int n = d->n, r = d->nDn, m = d->m;
     const long nn = (long) n * n;
@@ -731,15 +732,20 @@ void dense_gram_mixed(hdsdp_dev *d) {
         const __bf16 *Ap = d->Dall ? d->Dall + (size_t) b0 * nnp : d->Dp;
         const long psA = d->Dall ? nnp * r : bps;
         if (!d->Dall) hdk::unpack_dense_split(d->st, d->c, b0, cnt, d->Dp, ldn, bps, nnp);
-        // V = A_p L^-T: column j of L^-T is zero below row j (tri 1); written as planes
+        // With A_p = E_p + E_p^T (E_p the lower triangle of A_p, diagonal halved) and the
+        // lower-triangular U_p = L^-1 E_p, T_p = L^-1 A_p L^-T = U_p L^-T + (U_p L^-T)^T.
+        // U_p^T = E_p^T L^-T (upper triangular, k from i to j: tri 3) is written as planes
+        // (the tiles below the diagonal of the zero-filled buffer stay zero); then the
+        // lower triangle of T_p, entry (i, j) = sum_k U^T(k, i) L^-T(k, j) + L^-T(k, i) U^T(k, j),
+        // as one product over both sums (every operand column i zero below row i: tri 4),
+        // written as packed planes of its strictly lower part and of its diagonal, the
+        // operands of the Gram products below.  n^3 / 6 + n^3 / 3 multiply-adds per
+        // constraint, against n^3 / 2 + n^3 / 3 for V_p = A_p L^-T, T_p = L^-1 V_p.
         hdk::gemm_bf16p(d->st, false, n, n, ldn, 1.0, Ap, ldn, psA, nnp, d->LiTp, ldn, nnp, 0, 0.0, d->Vp,
-                        hdk::BF16P_OUT_PLANES, ldn, bps, nnp, cnt, 1);
-        // lower triangle of T_p = (L^-T)^T V only (symmetric; column i of the left operand
-        // L^-T is zero below row i, tri 2), written as packed planes of its strictly lower
-        // part and of its diagonal, the two operands of the Gram products below
-        hdk::gemm_bf16p_packed(d->st, n, ldn, 1.0, d->LiTp, ldn, nnp, 0, d->Vp, ldn, bps, nnp,
+                        hdk::BF16P_OUT_PLANES, ldn, bps, nnp, cnt, 3);
+        hdk::gemm_bf16p_packed(d->st, n, ldn, 1.0, d->Vp, ldn, bps, nnp, d->LiTp, ldn, nnp, 0,
                                d->Tp + (size_t) b0 * K1p, K1p, K1p * r, d->Tdgp + (size_t) b0 * ldn, ldn,
-                               (long) ldn * r, cnt, 2);
+                               (long) ldn * r, cnt, 4, d->LiTp, nnp, 0, d->Vp, bps, nnp);
     }
     const long kc = d->chunkP;
     const int full = (int) (K1p / kc);
