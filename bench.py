@@ -313,13 +313,14 @@ def main():
     # roofline of the dominant kernel of the headline path
     if sflops is not None and "schur_build" in hs and hprof["gram"][1]:
         sb = hs["schur_build"]
-        kern = "gemm_bf16p_kernel (bf16 plane products of the dense Schur build: V = A L^-T, " \
-            "T = L^-1 V, <T_p, T_q>)" if sb["mfma_products_per_flop"] > 1 else \
+        kern = "gemm_bf16p_kernel (bf16 plane products of the dense Schur build: U^T = E^T L^-T, " \
+            "T = U L^-T + (U L^-T)^T, <T_p, T_q>)" if sb["mfma_products_per_flop"] > 1 else \
             "gemm_kernel / gemm_big_kernel (FP64 Gram products)"
         roof = {
             "bound": "mfma", "kernel": kern, "achieved": sb["mfma_tflops"], "peak": sb["peak_tflops"],
             "unit": "TFLOP/s", "frac": sb["frac"],
-            "traffic": profile_value(f"dense1000_{head}_gram_traffic.md", "traffic_bytes_per_build"),
+            "traffic": profile_value(f"dense1000_{head}_gram_traffic.md", "traffic_bytes_per_build")
+            if args.workload == "C3" else None,
             "algorithmic_flops_per_build": sflops * sb["mfma_products_per_flop"],
             "avg_build_ms": sb["gram_avg_ms"],
         }

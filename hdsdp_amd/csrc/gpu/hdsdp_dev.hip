@@ -99,6 +99,8 @@ struct hdsdp_dev {
     double *precS;     // Jacobi scaling of the FP32 preconditioner
     double precShift;  // diagonal shift the FP32 factorisation needed
     int chol32Min;      // mixed: FP32 factor preconditioner for an FP64-built M once m >= this
+    double *hsdCoef;    // coefficient vector e_0 (C alone) for the self-dual embedding's terms
+    bool wFromDual;     // W holds S^-1 of the last Schur build (not the primal method's X)
     double mixedFloor;  // mixed: refinement against the exact operator stops at this fraction of ||b||
     double *cb, *cxa;   // mixed refinement: right-hand side and accumulated solution
     int precKind;
@@ -1354,7 +1356,8 @@ hdsdp_retcode hdsdp_dev_ratio_test(hdsdp_dev *d, int which, double *maxStep) {
 hdsdp_retcode hdsdp_dev_schur(hdsdp_dev *d, int typeKKT, double Rd, double *asinv, double *asinvrd,
                               double *traceSinv) {
     const int m = d->m, n = d->n;
-    if (typeKKT == 2) return HDSDP_RETCODE_FAILED;
+    // HOMOGENEOUS (2) builds what INFEASIBLE builds; its extra terms come from
+    // hdsdp_dev_hsd on the S^-1 formed here
     if (typeKKT == 3) {
         // PRIMAL: strategy M5 with W := X for every column (interface/hdsdp_conic_sdp.c:1741-1745,
         // 1813-1820); the reference's M5 takes no DENSE or DSR1 coefficient
@@ -1424,6 +1427,45 @@ hdsdp_retcode hdsdp_dev_schur(hdsdp_dev *d, int typeKKT, double Rd, double *asin
         DCALL(read_scalars(d));
         *traceSinv = d->hsc[hdk::SC_AUX];
     }
+    d->wFromDual = (typeKKT != 3);
+    return HDSDP_RETCODE_OK;
+}
+
+// The self-dual embedding's terms (sdpDenseConeIGetHSDComponents,
+// interface/hdsdp_conic_sdp.c:983-1029, and the dASinvCSinvVec entries that strategies
+// M1-M5 add for KKT_TYPE_HOMOGENEOUS, 737-760 and alike) from the W = S^-1 of the last
+// Schur build: with B = W C W,
+//   dASinvCSinv[i] = <A_i, B>,  dCSinv = <C, W>,  dCSinvCSinv = <C, B>,
+//   dCSinvRdSinv = Rd tr(B).
+hdsdp_retcode hdsdp_dev_hsd(hdsdp_dev *d, double Rd, double *aSinvCSinv, double *cSinv, double *cSinvCSinv,
+                            double *cSinvRdSinv) {
+    const int m = d->m, n = d->n;
+    if (!d->wFromDual) return HDSDP_RETCODE_FAILED;
+    if (!d->hsdCoef) DCALL(dalloc(d, &d->hsdCoef, (size_t) m + 1));
+    // C alone: the coefficient vector e_0
+    if (d->stagingBusy) DCALL(sync(d));
+    memset(d->hcoef, 0, sizeof(double) * (m + 1));
+    d->hcoef[0] = 1.0;
+    HIP_OK(hipMemcpyAsync(d->hsdCoef, d->hcoef, sizeof(double) * (m + 1), hipMemcpyHostToDevice, d->st));
+    d->stagingBusy = true;
+    hdk::assemble_buffer(d->st, d->c, d->hsdCoef, 1.0, d->T1, 0.0, d->work, d->splitW, d->splitLen);
+    hdk::mirror_lower<double>(d->st, n, d->T1, n);
+    hdk::gemm<double>(d->st, false, false, false, n, n, n, 1.0, d->W, n, 0, d->T1, n, 0, 0.0, d->T2, n, 0, 1);
+    hdk::gemm<double>(d->st, false, false, false, n, n, n, 1.0, d->T2, n, 0, d->W, n, 0, 0.0, d->T1, n, 0, 1);
+    // <A_k, B> for k = 0 (C) .. m, and <C, W>
+    hdk::traces(d->st, d->c, 0, m + 1, d->T1, nullptr, 0.0, d->vecA_d, nullptr, false);
+    hdk::r1_quadforms(d->st, d->c, d->T1, true, 0, m + 1, 0.0, d->vecA_d, nullptr, d->H1);
+    hdk::traces(d->st, d->c, 0, 1, d->W, nullptr, 0.0, d->vecB_d, nullptr, false);
+    hdk::r1_quadforms(d->st, d->c, d->W, true, 0, 1, 0.0, d->vecB_d, nullptr, d->H1);
+    hdk::trace_diag(d->st, n, d->T1, d->sc_d + hdk::SC_AUX);
+    DCALL(check_launch());
+    DCALL(download(d, d->hvecB, d->vecA_d, m + 1, d->hvecA));
+    memcpy(aSinvCSinv, d->hvecB + 1, sizeof(double) * m);
+    *cSinvCSinv = d->hvecB[0];
+    DCALL(download(d, d->hvecB, d->vecB_d, 1, d->hvecA));
+    *cSinv = d->hvecB[0];
+    DCALL(read_scalars(d));
+    *cSinvRdSinv = Rd * d->hsc[hdk::SC_AUX];
     return HDSDP_RETCODE_OK;
 }
 

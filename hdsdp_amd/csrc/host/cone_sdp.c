@@ -176,11 +176,6 @@ hdsdp_retcode sdp_cone_schur( sdp_cone *cone, host_kkt *kkt, int typeKKT ) {
     double *asinv = NULL, *asinvrd = NULL;
     double traceSinv = 0.0;
     int m = cone->nRow;
-    if ( typeKKT == KKT_TYPE_HOMOGENEOUS ) {
-        /* The self-dual embedding is not carried by this build. */
-        retcode = HDSDP_RETCODE_FAILED;
-        goto exit_cleanup;
-    }
     HALLOC(asinv, double, m);
     HALLOC(asinvrd, double, m);
     HCALL(hdsdp_dev_schur(cone->dev, typeKKT, cone->dualResidual, asinv, asinvrd, &traceSinv));
@@ -189,6 +184,15 @@ hdsdp_retcode sdp_cone_schur( sdp_cone *cone, host_kkt *kkt, int typeKKT ) {
         kkt->dASinvRdSinvVec[i] += asinvrd[i];
     }
     kkt->dTraceSinv += traceSinv;
+    if ( typeKKT == KKT_TYPE_HOMOGENEOUS ) {
+        /* the embedding's C terms (interface/hdsdp_conic_sdp.c:737-760, 983-1029) */
+        double cs = 0.0, csc = 0.0, csrd = 0.0;
+        HCALL(hdsdp_dev_hsd(cone->dev, cone->dualResidual, asinv, &cs, &csc, &csrd));
+        for ( int i = 0; i < m; ++i ) kkt->dASinvCSinvVec[i] += asinv[i];
+        kkt->dCSinv += cs;
+        kkt->dCSinvCSinv += csc;
+        if ( cone->dualResidual ) kkt->dCSinvRdSinv += csrd;
+    }
 exit_cleanup:
     free(asinv);
     free(asinvrd);

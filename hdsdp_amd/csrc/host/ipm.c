@@ -5,6 +5,8 @@
  * the infeasible corrector (interface/hdsdp_algo.c:959-1203, 666-957).
  * Phase B: dual potential reduction with barrier choice, ray detection and the
  * feasible corrector (interface/hdsdp_algo.c:1205-1846).
+ * Self-dual embedding: run when phase A stops with suspected infeasibility
+ * (interface/hdsdp_algo.c:262-545, 1859-1875).
  * The host keeps every control decision of the reference; each matrix
  * operation is one call into the SDP cone (cone_sdp.c -> device ABI).
  *
@@ -62,7 +64,11 @@ static void reset_start( hdsdp *s ) {
 }
 
 static void print_header( hdsdp *s, int method ) {
-    if ( method == ALGO_DUAL_INFEAS ) {
+    if ( method == ALGO_DUAL_HSD ) {
+        hlog(s, "HDSDP starts. Using self-dual method \n\n");
+        hlog(s, "    %5s  %15s  %15s  %8s  %8s  %5s  %6s   %5s \n",
+             "nIter", "pObj", "dObj", "dInf", "Mu", "Step", "Tau", "T [H]");
+    } else if ( method == ALGO_DUAL_INFEAS ) {
         hlog(s, "HDSDP starts. Using infeasible dual method \n\n");
         hlog(s, "    %5s  %15s  %15s  %8s  %8s  %5s  %6s   %5s \n",
              "nIter", "pObj", "dObj", "dInf", "Mu", "Step", "|P|", "T [D]");
@@ -78,12 +84,12 @@ void hdsdp_record_log( hdsdp *s, int phase ) {
     double *r = s->iterLog + (size_t) s->nLog * HDSDP_LOG_FIELDS;
     r[0] = phase;
     r[1] = s->nIterCount + 1;
-    r[2] = s->pObjVal;
+    r[2] = ( phase == ALGO_DUAL_HSD ) ? HDSDP_INFINITY : s->pObjVal;
     r[3] = s->dObjVal;
     r[4] = ( phase == ALGO_DUAL_POTENTIAL || phase == ALGO_PRIMAL ) ? s->pInfeas : s->dInfeas;
     r[5] = s->dBarrierMu;
     r[6] = s->dDStep;
-    r[7] = s->dProxNorm;
+    r[7] = ( phase == ALGO_DUAL_HSD ) ? s->dBarHsdTau : s->dProxNorm;
     r[8] = hdsdp_timestamp() - s->dTimeBegin;
     s->nLog += 1;
 }
@@ -100,9 +106,14 @@ static void print_log( hdsdp *s, int method ) {
     s->dObjVal = s->dObjInternal * pdObjScal;
     s->pObjVal = s->pObjInternal * pdObjScal;
     s->comp = s->pObjVal - s->dObjVal;
-    hlog(s, "    %5d  %+15.8e  %+15.8e  %8.2e  %8.2e  %5.2f  %5.1e  %4.1f \n", s->nIterCount + 1,
-         s->pObjVal, s->dObjVal, method == ALGO_DUAL_POTENTIAL ? s->pInfeas : s->dInfeas,
-         s->dBarrierMu, s->dDStep, s->dProxNorm, elapsed);
+    if ( method == ALGO_DUAL_HSD ) {
+        hlog(s, "    %5d  %+15.8e  %+15.8e  %8.2e  %8.2e  %5.2f  %5.1e  %4.1f\n", s->nIterCount + 1,
+             HDSDP_INFINITY, s->dObjVal, s->dInfeas, s->dBarrierMu, s->dDStep, s->dBarHsdTau, elapsed);
+    } else {
+        hlog(s, "    %5d  %+15.8e  %+15.8e  %8.2e  %8.2e  %5.2f  %5.1e  %4.1f \n", s->nIterCount + 1,
+             s->pObjVal, s->dObjVal, method == ALGO_DUAL_POTENTIAL ? s->pInfeas : s->dInfeas,
+             s->dBarrierMu, s->dDStep, s->dProxNorm, elapsed);
+    }
     hdsdp_record_log(s, method);
 }
 
@@ -871,14 +882,173 @@ exit_cleanup:
     return retcode;
 }
 
-/* interface/hdsdp_algo.c:1861-1900 */
+/* interface/hdsdp_algo.c:73-82: the self-dual start (used when the embedding runs first) */
+static void set_start_hsd( hdsdp *s, int dOnly ) {
+    memset(s->dRowDual, 0, sizeof(double) * s->nRows);
+    s->dBarHsdTau = 1.0;
+    double objFroNorm = get_dbl_feature(s, DBL_FEATURE_OBJFRONORM) * get_dbl_feature(s, DBL_FEATURE_OBJSCALING);
+    objFroNorm = HMAX(objFroNorm, 100.0);
+    s->dBarrierMu = 1e+08;
+    s->dResidual = dOnly ? -objFroNorm * get_dbl_param(s, DBL_PARAM_DUALSTART) : -objFroNorm * 1e+01;
+    hlog(s, "Initialize with dual residual %3.1e\n", -s->dResidual);
+    s->sdp->dualResidual = s->dResidual;
+}
+
+/* interface/hdsdp_algo.c:262-313 (HDSDP_HSD_BuildStep): the tau step from the bordered
+ * system and dy = d1 (tau + dtau) / mu + d4 dtau - d2 + d3, with d1 = M^-1 b,
+ * d2 = M^-1 A S^-1, d3 = M^-1 A S^-1 Rd S^-1, d4 = M^-1 A S^-1 C S^-1 */
+static void hsd_build_step( hdsdp *s ) {
+    int m = s->nRows;
+    const double *b = s->rowRHS, *d1 = s->dMinvRowRHS, *d2 = s->dMinvASinv, *d3 = s->dMinvASinvRdSinv,
+                 *d4 = s->dMinvASinvCSinv, *aCS = s->kkt->dASinvCSinvVec;
+    double *dd1 = s->dHAuxiVec1, *dy = s->dRowDualStep;
+    double mu = s->dBarrierMu, tau = s->dBarHsdTau;
+    s->dBarHsdTauStep = 0.0;
+    double oldObj = s->dObjInternal;
+    s->dObjInternal = vdot(m, b, s->dRowDual);
+    s->dObjImprove = s->dObjInternal - oldObj;
+    for ( int i = 0; i < m; ++i ) dd1[i] = b[i] - mu * aCS[i];
+    double num = -s->dObjInternal + mu / tau + mu * (s->kkt->dCSinv - s->kkt->dCSinvRdSinv);
+    double den = mu * s->kkt->dCSinvCSinv + mu / (tau * tau);
+    double tauOverMu = tau / mu;
+    for ( int i = 0; i < m; ++i ) {
+        num -= dd1[i] * (d1[i] * tauOverMu - d2[i] + d3[i]);
+        den += dd1[i] * (d1[i] / mu + d4[i]);
+    }
+    double dtau = ( fabs(den) < 1e-12 ) ? 0.0 : num / den;
+    s->dBarHsdTauStep = dtau;
+    for ( int i = 0; i < m; ++i ) dy[i] = d1[i] * (tau + dtau) / mu + d4[i] * dtau - d2[i] + d3[i];
+}
+
+/* interface/hdsdp_algo.c:315-352 (HDSDP_HSD_RatioTest): tau and the SDP cone, no bound cone */
+static hdsdp_retcode hsd_ratio_test( hdsdp *s, double *maxDist ) {
+    hdsdp_retcode retcode = HDSDP_RETCODE_OK;
+    double maxStep = HDSDP_INFINITY, step = s->dBarHsdTau / s->dBarHsdTauStep;
+    if ( step < 0.0 ) maxStep = HMIN(maxStep, -step);
+    HCALL(sdp_cone_ratio_test(s->sdp, s->dBarHsdTauStep, s->dRowDualStep, 1.0, BUFFER_DUALVAR, &step));
+    maxStep = HMIN(maxStep, step);
+    if ( maxStep < 1e-02 ) {
+        s->nSmallStep += 1;
+        if ( s->nSmallStep > 2 ) hlog(s, "HDSDP stagates at the cone boundary. \n");
+    }
+exit_cleanup:
+    *maxDist = maxStep;
+    return retcode;
+}
+
+/* interface/hdsdp_algo.c:354-545 (HDSDP_PhaseA_BarHsdSolve): the self-dual embedding
+ * (serves as a certificate of dual (in)feasibility; no primal solution is extracted) */
+static hdsdp_retcode phase_hsd( hdsdp *s, int dOnly ) {
+    hdsdp_retcode retcode = HDSDP_RETCODE_OK;
+    int m = s->nRows;
+    s->whichMethod = ALGO_DUAL_HSD;
+    int nMaxIter = get_int_param(s, INT_PARAM_MAXITER);
+    double gamma = get_dbl_param(s, DBL_PARAM_HSDGAMMA);
+    double absFeas = get_dbl_param(s, DBL_PARAM_ABSFEASTOL), absOpt = get_dbl_param(s, DBL_PARAM_ABSOPTTOL);
+    double relFeas = get_dbl_param(s, DBL_PARAM_RELFEASTOL), relOpt = get_dbl_param(s, DBL_PARAM_RELOPTTOL);
+    double timeLimit = get_dbl_param(s, DBL_PARAM_TIMELIMIT);
+    double objOneNorm = get_dbl_feature(s, DBL_FEATURE_OBJONENORM);
+    double objScal = get_dbl_feature(s, DBL_FEATURE_OBJSCALING);
+    double nSumCones = (double) get_int_feature(s, INT_FEATURE_N_SUMCONEDIMS);
+    if ( !dOnly ) {
+        absOpt = 1e+20;
+        relOpt = 1e+20;
+    }
+    (void) relOpt;
+    double feasTol = HMIN(absFeas, relFeas * (1.0 + objOneNorm));
+    feasTol = feasTol * objScal / sqrt(nSumCones);
+    absOpt = absOpt * 1e-04;
+    relOpt = absOpt * 1e-04;  /* (from the absolute tolerance, as the reference sets it) */
+    if ( s->HStatus == HDSDP_UNKNOWN ) set_start_hsd(s, dOnly);
+    print_header(s, ALGO_DUAL_HSD);
+
+    while ( 1 ) {
+        int isInterior = 0;
+        HCALL(sdp_cone_interior(s->sdp, s->dBarHsdTau, s->dRowDual, &isInterior));
+        if ( !isInterior ) {
+            if ( s->nIterCount == 0 ) {
+                hlog(s, "Initial point is not in the cone. Adding slack value.\n");
+                s->dResidual *= 100.0;
+                reset_start(s);
+                s->nIterCount += 1;
+                continue;
+            }
+            hlog(s, "Iteration %d is not in the cone. \n", s->nIterCount);
+            retcode = HDSDP_RETCODE_FAILED;
+            goto exit_cleanup;
+        }
+        /* HKKTBuildUp over the SDP cone (no bound cone in the embedding); the
+         * HKKTRegularize(.., 0.0) that follows adds nothing */
+        HCALL(kkt_build(s, KKT_TYPE_HOMOGENEOUS));
+        memcpy(s->dMinvASinv, s->kkt->dASinvVec, sizeof(double) * m);
+        memcpy(s->dMinvASinvRdSinv, s->kkt->dASinvRdSinvVec, sizeof(double) * m);
+        memcpy(s->dMinvASinvCSinv, s->kkt->dASinvCSinvVec, sizeof(double) * m);
+        HCALL(kkt_factorize(s));
+        HCALL(kkt_solve(s, s->rowRHS, s->dMinvRowRHS));
+        HCALL(kkt_solve(s, s->dMinvASinv, s->dMinvASinv));
+        HCALL(kkt_solve(s, s->dMinvASinvRdSinv, s->dMinvASinvRdSinv));
+        HCALL(kkt_solve(s, s->dMinvASinvCSinv, s->dMinvASinvCSinv));
+        hsd_build_step(s);
+        HCALL(hsd_ratio_test(s, &s->dDStep));
+        if ( s->dDStep > 1.0 ) {
+            s->dDStep = HMIN(0.7 * s->dDStep, 1.0);
+        } else if ( s->dDStep > 0.5 ) {
+            s->dDStep = HMIN(0.5 * s->dDStep, 1.0);
+        } else if ( s->dDStep > 0.2 ) {
+            s->dDStep = HMIN(0.3 * s->dDStep, 1.0);
+        } else {
+            s->dDStep = HMIN(0.2 * s->dDStep, 1.0);
+        }
+        print_log(s, ALGO_DUAL_HSD);
+        s->dBarHsdTau += s->dDStep * s->dBarHsdTauStep;
+        for ( int i = 0; i < m; ++i ) s->dRowDual[i] += s->dDStep * s->dRowDualStep[i];
+        s->dResidual = s->dResidual * (1.0 - s->dDStep);
+        set_residual(s, s->dResidual);
+        double target = 0.0;
+        if ( s->dBarrierMu > 1e-12 ) {
+            if ( s->dDStep > 0.8 && s->dBarHsdTau > 1.0 ) {
+                target = 0.1 * s->dBarrierMu;
+            } else {
+                target = gamma * s->dBarrierMu;
+            }
+            target = HMAX(target, -0.1 * s->dResidual / s->dBarHsdTau);
+            s->dBarrierMu = HMIN(s->dBarrierMu, target);
+        } else {
+            target = 0.8 * s->dBarrierMu;
+            s->dBarrierMu = HMIN(s->dBarrierMu, target);
+        }
+        if ( fabs(s->dResidual) < feasTol * s->dBarHsdTau && s->dBarrierMu < absOpt &&
+             (s->dBarrierMu < relOpt * (1 + 2.0 * fabs(s->dObjVal)) &&
+              fabs(s->dObjImprove) < 1e-05 * (fabs(s->dObjInternal) + 1.0)) ) {
+            s->HStatus = dOnly ? HDSDP_DUAL_OPTIMAL : HDSDP_DUAL_FEASIBLE;
+            break;
+        }
+        if ( s->dBarHsdTau <= 1e-10 ) {
+            s->HStatus = HDSDP_SUSPECT_INFEAS_OR_UNBOUNDED;
+            break;
+        }
+        if ( hdsdp_timestamp() - s->dTimeBegin >= timeLimit ) {
+            s->HStatus = HDSDP_TIMELIMIT;
+            break;
+        }
+        s->nIterCount += 1;
+        if ( s->nIterCount >= nMaxIter ) {
+            s->HStatus = HDSDP_MAXITER;
+            break;
+        }
+    }
+exit_cleanup:
+    if ( retcode != HDSDP_RETCODE_OK ) s->HStatus = HDSDP_NUMERICAL;
+    return retcode;
+}
+
+/* interface/hdsdp_algo.c:1859-1900 */
 hdsdp_retcode hdsdp_solve( hdsdp *s, int dOnly ) {
     hdsdp_retcode retcode = HDSDP_RETCODE_OK;
-    (void) dOnly;
     HCALL(phase_a(s));
     if ( s->HStatus == HDSDP_SUSPECT_INFEAS_OR_UNBOUNDED ) {
         hlog(s, "\nInfeasible method stops due to suspected infeasibility \n");
-        hlog(s, "The self-dual embedding phase is not carried by this build.\n");
+        HCALL(phase_hsd(s, dOnly));
     } else if ( s->HStatus == HDSDP_DUAL_FEASIBLE ) {
         hlog(s, "\nInfeasible method finds a dual feasible solution \n");
         HCALL(phase_b(s));

@@ -56,7 +56,9 @@ hdsdp_retcode kkt_build( hdsdp *s, int typeKKT ) {
     host_kkt *kkt = s->kkt;
     memset(kkt->dASinvVec, 0, sizeof(double) * kkt->nRow);
     memset(kkt->dASinvRdSinvVec, 0, sizeof(double) * kkt->nRow);
+    memset(kkt->dASinvCSinvVec, 0, sizeof(double) * kkt->nRow);
     kkt->dTraceSinv = 0.0;
+    kkt->dCSinv = kkt->dCSinvCSinv = kkt->dCSinvRdSinv = 0.0;
     return sdp_cone_schur(s->sdp, kkt, typeKKT);
 }
 

@@ -103,12 +103,22 @@ hdsdp_retcode hdsdp_dev_ratio_test( hdsdp_dev *dev, int whichBuffer, double *max
  *   M_ij = <A_i, W A_j W> (lower triangle, not for CORRECTOR),
  *   dASinv[i] = <A_i, W>, dASinvRdSinv[i] = Rd <A_i, W W>,
  *   *dTraceSinv = tr(W) when Rd != 0 (not for CORRECTOR), else 0.
- * HOMOGENEOUS (2, the HSD scalars) returns HDSDP_RETCODE_FAILED: the host runs
- * without the HSD phase.
+ * HOMOGENEOUS (2) builds the same; the self-dual embedding's extra terms then come
+ * from hdsdp_dev_hsd.
  * Replaces sdpDenseConeGetKKT and its strategies M2-M5
  * (interface/hdsdp_conic_sdp.c:683-1052, 1722-1872). */
 hdsdp_retcode hdsdp_dev_schur( hdsdp_dev *dev, int typeKKT, double dualResidual,
                                double *dASinv, double *dASinvRdSinv, double *dTraceSinv );
+
+/* The self-dual embedding's terms of a HOMOGENEOUS build, from its W = S^-1 (call
+ * after hdsdp_dev_schur with typeKKT 0 or 2, before the dual factor changes): with
+ * B = W C W, dASinvCSinv[i] = <A_i, B> (m entries), *dCSinv = <C, W>,
+ * *dCSinvCSinv = <C, B>, *dCSinvRdSinv = dualResidual tr(B).
+ * Replaces sdpDenseConeIGetHSDComponents (interface/hdsdp_conic_sdp.c:983-1029) and
+ * the dASinvCSinvVec entries of strategies M1-M5 (interface/hdsdp_conic_sdp.c:737-760
+ * and the alike blocks of M2-M5).  HDSDP_RETCODE_FAILED when the last build was PRIMAL. */
+hdsdp_retcode hdsdp_dev_hsd( hdsdp_dev *dev, double dualResidual, double *dASinvCSinv,
+                             double *dCSinv, double *dCSinvCSinv, double *dCSinvRdSinv );
 
 /* ---- Schur solve (hdsdp_linsys_fp slots of the KKT solver) -------------- */
 

@@ -13,6 +13,7 @@
  *   CG on M              linalg/hdsdp_linsolver.c:698-1069
  *   Lanczos step length  linalg/hdsdp_lanczos.c:33-292, interface/hdsdp_conic_sdp.c:462-505, 1638-1682
  *   primal recovery      interface/hdsdp_conic_sdp.c:2328-2379
+ *   self-dual terms      interface/hdsdp_conic_sdp.c:737-760, 983-1029
  *
  * Pinning: see tests/test_oracle.py (closed forms, brute-force traces on tiny
  * inputs, numpy/scipy factorizations) and the reference golden traces.
@@ -61,6 +62,7 @@ struct hdsdp_dev {
     double *Xs, *dXp, *XSX, *Sold, *LX;
     olanczos lzP;
     int psdpReady;
+    int wFromDual;  /* W holds S^-1 of the last Schur build (not X) */
     double stats[6];
 };
 
@@ -611,7 +613,8 @@ hdsdp_retcode hdsdp_dev_schur( hdsdp_dev *d, int typeKKT, double Rd, double *asi
                                double *traceSinv ) {
     int n = d->n, m = d->m;
     size_t nn = (size_t) n * n;
-    if ( typeKKT == 2 ) return HDSDP_RETCODE_FAILED;
+    /* HOMOGENEOUS (2) builds what INFEASIBLE builds; its extra terms: hdsdp_dev_hsd */
+    d->wFromDual = ( typeKKT != 3 );
     if ( typeKKT == 3 ) {
         /* PRIMAL: strategy M5 with W := X (interface/hdsdp_conic_sdp.c:1741-1745, 1813-1820);
          * no DENSE or DSR1 constraint (linalg/hdsdp_sdpdata.c:1961-1965, 1995-1999) */
@@ -655,6 +658,28 @@ hdsdp_retcode hdsdp_dev_schur( hdsdp_dev *d, int typeKKT, double Rd, double *asi
             FE(d->M, m, i, j) = v;
         }
     }
+    free(B);
+    return HDSDP_RETCODE_OK;
+}
+
+/* The self-dual embedding's terms (interface/hdsdp_conic_sdp.c:983-1029,
+ * sdpDenseConeIGetHSDComponents, and the dASinvCSinvVec entries strategies M1-M5 add
+ * for KKT_TYPE_HOMOGENEOUS, 737-760 and the alike blocks): with B = W C W and W = S^-1
+ * of the last build, dASinvCSinv[i] = <A_i, B>, dCSinv = <C, W>, dCSinvCSinv = <C, B>,
+ * dCSinvRdSinv = Rd tr(B). */
+hdsdp_retcode hdsdp_dev_hsd( hdsdp_dev *d, double Rd, double *aSinvCSinv, double *cSinv, double *cSinvCSinv,
+                             double *cSinvRdSinv ) {
+    int n = d->n, m = d->m;
+    if ( !d->wFromDual ) return HDSDP_RETCODE_FAILED;
+    double *B = (double *) malloc(sizeof(double) * (size_t) n * n);
+    if ( !B ) return HDSDP_RETCODE_MEMORY;
+    osandwich(&d->A[0], n, d->W, d->tmp, d->tmp2, B);
+    for ( int i = 0; i < m; ++i ) aSinvCSinv[i] = otrace_AB(&d->A[i + 1], n, B, d->vecn);
+    *cSinv = otrace_AB(&d->A[0], n, d->W, d->vecn);
+    *cSinvCSinv = otrace_AB(&d->A[0], n, B, d->vecn);
+    double tr = 0.0;
+    for ( int j = 0; j < n; ++j ) tr += FE(B, n, j, j);
+    *cSinvRdSinv = Rd * tr;
     free(B);
     return HDSDP_RETCODE_OK;
 }

@@ -194,6 +194,12 @@ class Dev:
         self._ok(self.lib.hdsdp_dev_schur(self.h, typeKKT, Rd, _capi.dptr(a), _capi.dptr(b), C.byref(t)), "schur")
         return a, b, t.value
 
+    def hsd(self, Rd):
+        a = np.zeros(self.m)
+        cs, csc, csrd = C.c_double(), C.c_double(), C.c_double()
+        self._ok(self.lib.hdsdp_dev_hsd(self.h, Rd, _capi.dptr(a), C.byref(cs), C.byref(csc), C.byref(csrd)), "hsd")
+        return a, cs.value, csc.value, csrd.value
+
     def get_schur(self):
         M = np.zeros((self.m, self.m), order="F")
         self._ok(self.lib.hdsdp_dev_get_schur(self.h, M.ctypes.data_as(_capi.P_DBL)), "get_schur")

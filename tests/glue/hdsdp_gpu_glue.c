@@ -30,8 +30,8 @@
  * bit, with two additions (-d, then x).
  *
  * Supported: one DENSE_SDP cone, a dense Schur matrix, the dual-scaling
- * iterations (KKT types INFEASIBLE and CORRECTOR).  PSDP (KKT_TYPE_PRIMAL) and the
- * embedding (HOMOGENEOUS) return HDSDP_RETCODE_FAILED.  Device precision:
+ * iterations (KKT types INFEASIBLE and CORRECTOR) and the self-dual embedding
+ * (HOMOGENEOUS).  PSDP (KKT_TYPE_PRIMAL) returns HDSDP_RETCODE_FAILED.  Device precision:
  * HDSDP_GLUE_PRECISION=0 (FP64, default) or 1 (mixed).
  *
  * At exit one line "GLUE {...}" reports how many calls went to the device.
@@ -87,9 +87,9 @@ static long gCalls[8];           /* update, factorize, ratio, schur, kktfac, kkt
 static int gShadow = 0;
 static hdsdp_cone gRef;          /* the DENSE_SDP slot table as the reference filled it */
 static hdsdp_linsys_fp gRefLin;  /* the reference's Schur solver kind */
-enum { SH_RATIO, SH_LOGDET, SH_ASINV, SH_ASINVRD, SH_TRACE, SH_M, SH_SOLVE, SH_X, SH_N };
+enum { SH_RATIO, SH_LOGDET, SH_ASINV, SH_ASINVRD, SH_TRACE, SH_M, SH_SOLVE, SH_X, SH_HSD, SH_N };
 static const char *gShName[SH_N] = {"ratio_test", "logdet", "asinv", "asinvrd", "trace_sinv", "schur_m",
-                                    "kkt_solve", "primal_x"};
+                                    "kkt_solve", "primal_x", "hsd_terms"};
 static double gShMax[SH_N];      /* largest relative difference, device vs reference */
 static int gShAt[SH_N];          /* the call at which it occurred */
 static long gShCount[SH_N];
@@ -279,14 +279,16 @@ static hdsdp_retcode glue_schur( void *cone, void *kkt, int typeKKT ) {
     glue_cone *g = glue_of(cone);
     hdsdp_kkt *K = (hdsdp_kkt *) kkt;
     const int m = g->host->nRow;
-    double *asinv = NULL, *asinvrd = NULL, tr = 0.0;
-    if ( typeKKT != KKT_TYPE_INFEASIBLE && typeKKT != KKT_TYPE_CORRECTOR ) {
+    double *asinv = NULL, *asinvrd = NULL, *acs = NULL, tr = 0.0, cs = 0.0, csc = 0.0, csrd = 0.0;
+    const int hsd = ( typeKKT == KKT_TYPE_HOMOGENEOUS );
+    if ( typeKKT != KKT_TYPE_INFEASIBLE && typeKKT != KKT_TYPE_CORRECTOR && !hsd ) {
         fprintf(stderr, "[glue] Schur type %d is not forwarded\n", typeKKT);
         return HDSDP_RETCODE_FAILED;
     }
     asinv = (double *) calloc((size_t) m, sizeof(double));
     asinvrd = (double *) calloc((size_t) m, sizeof(double));
-    if ( !asinv || !asinvrd ) {
+    acs = (double *) calloc((size_t) m, sizeof(double));
+    if ( !asinv || !asinvrd || !acs ) {
         retcode = HDSDP_RETCODE_MEMORY;
         goto exit_cleanup;
     }
@@ -296,6 +298,9 @@ static hdsdp_retcode glue_schur( void *cone, void *kkt, int typeKKT ) {
         GCALL(hdsdp_dev_get_schur(g->dev, g->mFull));
         for ( int i = 0; i < m; ++i ) g->mDiag[i] = g->mFull[(size_t) i * m + i];
     }
+    /* the embedding's C terms (sdpDenseConeIGetHSDComponents and the dASinvCSinvVec
+     * entries, interface/hdsdp_conic_sdp.c:737-760, 983-1029) */
+    if ( hsd ) GCALL(hdsdp_dev_hsd(g->dev, g->host->dualResidual, acs, &cs, &csc, &csrd));
     if ( gShadow ) {
         /* the reference's build adds into zeroed vectors and, for a new M, a zeroed matrix */
         for ( int i = 0; i < m; ++i ) {
@@ -305,8 +310,13 @@ static hdsdp_retcode glue_schur( void *cone, void *kkt, int typeKKT ) {
                 goto exit_cleanup;
             }
         }
-        const double tr0 = K->dTraceSinv;
+        const double tr0 = K->dTraceSinv, cs0 = K->dCSinv, csc0 = K->dCSinvCSinv;
         GCALL(gRef.coneBuildSchur(cone, kkt, typeKKT));
+        if ( hsd ) {
+            shadow_note(SH_HSD, rel_dist(m, acs, K->dASinvCSinvVec));
+            shadow_note(SH_HSD, rel_scalar(cs0 + cs, K->dCSinv));
+            shadow_note(SH_HSD, rel_scalar(csc0 + csc, K->dCSinvCSinv));
+        }
         shadow_note(SH_ASINV, rel_dist(m, asinv, K->dASinvVec));
         if ( g->host->dualResidual != 0.0 ) shadow_note(SH_ASINVRD, rel_dist(m, asinvrd, K->dASinvRdSinvVec));
         if ( typeKKT != KKT_TYPE_CORRECTOR ) {
@@ -333,9 +343,16 @@ static hdsdp_retcode glue_schur( void *cone, void *kkt, int typeKKT ) {
     if ( typeKKT != KKT_TYPE_CORRECTOR ) {
         for ( int i = 0; i < m; ++i ) *K->kktDiag[i] += g->mDiag[i];
     }
+    if ( hsd ) {
+        for ( int i = 0; i < m; ++i ) K->dASinvCSinvVec[i] += acs[i];
+        K->dCSinv += cs;
+        K->dCSinvCSinv += csc;
+        if ( g->host->dualResidual ) K->dCSinvRdSinv += csrd;
+    }
 exit_cleanup:
     free(asinv);
     free(asinvrd);
+    free(acs);
     return retcode;
 }
 

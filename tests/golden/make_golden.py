@@ -66,6 +66,10 @@ CASES = {
     "dense_n200_m400": (lambda: I.dense_random(200, 400, 34), {"relopt": 1e-6, "absopt": 1e30}, {"medium"}),
     "rankone_n200_m2000": (lambda: I.rank_one(200, 2000, 20, 4, form="csc"), {"relopt": 1e-6, "absopt": 1e30},
                            {"heavy"}),
+    # C = -5 I leaves no interior dual point: phase A stops with suspected infeasibility and
+    # the self-dual embedding runs until tau vanishes (interface/hdsdp_algo.c:1859-1875)
+    "hsd_dense_n10_m15": (lambda: I.with_data(I.dense_random(10, 15, 7), C=-np.eye(10) * 5.0), {"relopt": 1e-6},
+                          set()),
     # the primal method on (it starts once the dual gap is below 10% and ends the solve)
     "theta_n50_psdp": (lambda: I.lovasz_theta(50, 0.3, 21), {"psdp": 1}, set()),
     "maxcut_n60_psdp": (lambda: I.maxcut(60, 0.2, 22, "unif"), {"psdp": 1}, {"dense_primary"}),
@@ -111,6 +115,8 @@ def run_reference(inst, opts, timeout=3000, runner=RUNNER, cbwr=""):
     for line in out.splitlines():
         if "infeasible dual method" in line:
             phase = 0
+        elif "self-dual method" in line:
+            phase = 1
         elif "feasible dual method" in line:
             phase = 2
         elif "Primal solver is initialized" in line:

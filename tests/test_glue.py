@@ -56,10 +56,11 @@ def check_forwarded(r, calls, out):
     assert calls["kkt_factorizations"] >= r.iters
     assert calls["kkt_solves"] >= 2 * r.iters
     assert calls["factorizations"] > 0 and calls["ratio_tests"] > 0 and calls["updates"] > 0
-    assert calls["primal_recoveries"] >= 1
+    if r.status != 5:  # a solve that ends with suspected infeasibility recovers no X
+        assert calls["primal_recoveries"] >= 1
 
 
-@pytest.mark.parametrize("name", ["c1_gap1e-6", "maxcut_n60", "theta_n50", "dense_n40_m60"])
+@pytest.mark.parametrize("name", ["c1_gap1e-6", "maxcut_n60", "theta_n50", "dense_n40_m60", "hsd_dense_n10_m15"])
 def test_reference_host_over_oracle_abi(name):
     r, calls, out, fix = run_glue(GLUE_ORACLE, name)
     check_forwarded(r, calls, out)
@@ -67,9 +68,9 @@ def test_reference_host_over_oracle_abi(name):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["c1_gap1e-6", "c2_gap1e-6"])
+@pytest.mark.parametrize("name", ["c1_gap1e-6", "c2_gap1e-6", "hsd_dense_n10_m15"])
 def test_reference_host_over_device_abi(name):
-    """C1 and C2 through the reference's unmodified hdsdp_algo.c with the device slots."""
+    """C1, C2 and a self-dual embedding run through the reference's unmodified hdsdp_algo.c with the device slots."""
     r, calls, out, fix = run_glue(GLUE_GPU, name)
     check_forwarded(r, calls, out)
     dt.check_parity(fix, r, f"glue/gpu {name}")
@@ -85,7 +86,7 @@ def test_reference_host_over_device_abi(name):
 # the device's own previous call left, and the late Schur systems are ill-conditioned
 # enough that two CG runs to the same residual tolerance differ in their solutions.
 SHADOW_BOUNDS = {"schur_m": 1e-5, "asinv": 1e-6, "asinvrd": 1e-6, "trace_sinv": 1e-10, "logdet": 1e-8,
-                 "primal_x": 1e-6}
+                 "primal_x": 1e-6, "hsd_terms": 1e-6}
 
 
 def check_shadow(out, r, fix):
@@ -102,14 +103,16 @@ def check_shadow(out, r, fix):
     assert sh["schur_m"][2] >= r.iters and sh["ratio_test"][2] > 0 and sh["kkt_solve"][2] > 0
 
 
-@pytest.mark.parametrize("name", ["c1_gap1e-6", "theta_n50", "dense_n40_m60", "rankone_n60_m500"])
+@pytest.mark.parametrize("name", ["c1_gap1e-6", "theta_n50", "dense_n40_m60", "rankone_n60_m500",
+                                  "hsd_dense_n10_m15"])
 def test_oracle_abi_at_reference_iterates(name):
     r, calls, out, fix = run_glue(GLUE_ORACLE, name, shadow=True)
     check_shadow(out, r, fix)
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["c1_gap1e-6", "c2_gap1e-6", "dense_n120_m240", "rankone_n60_m500"])
+@pytest.mark.parametrize("name", ["c1_gap1e-6", "c2_gap1e-6", "dense_n120_m240", "rankone_n60_m500",
+                                  "hsd_dense_n10_m15"])
 def test_device_abi_at_reference_iterates(name):
     """Every device slot call of a reference run, against the reference's own answer."""
     r, calls, out, fix = run_glue(GLUE_GPU, name, shadow=True)

@@ -85,6 +85,30 @@ def test_schur_matches_oracle_and_closed_form(gpu_lib, oracle, name):
     assert rel(g.get_schur(), gM) == 0.0
 
 
+@pytest.mark.parametrize("name", list(CASES))
+def test_hsd_terms_match_oracle_and_closed_form(gpu_lib, oracle, name):
+    """The self-dual embedding's terms of a HOMOGENEOUS build (B = W C W):
+    <A_i, B>, <C, W>, <C, B> and Rd tr(B) against numpy and the oracle."""
+    coeffs, g, o, rng = make_pair(gpu_lib, oracle, name, seed=5)
+    y, eye = dt.interior_point(coeffs, rng)
+    Rd = -0.37
+    res = []
+    for d in (g, o):
+        d.update(1.0, -1.0, y, eye, 0)
+        assert d.factorize(0) == 1
+        d.schur(2, Rd)
+        res.append(d.hsd(Rd))
+    (ga, gcs, gcsc, grd), (oa, ocs, ocsc, ord_) = res
+    W = np.linalg.inv(dt.dense_buffer(coeffs, 1.0, -1.0, y, eye))
+    Cm = coeffs[0].dense()
+    B = W @ Cm @ W
+    A = [c.dense() for c in coeffs[1:]]
+    assert rel(ga, [np.sum(Ai * B) for Ai in A]) < 1e-9 and rel(ga, oa) < 1e-10
+    for got, orc, ref in ((gcs, ocs, np.sum(Cm * W)), (gcsc, ocsc, np.sum(Cm * B)), (grd, ord_, Rd * np.trace(B))):
+        scale = max(1.0, abs(ref), np.abs(Cm).max() * np.abs(W).sum())
+        assert abs(got - ref) <= 1e-10 * scale and abs(got - orc) <= 1e-11 * scale
+
+
 @pytest.mark.parametrize("name", ["spr1_unit", "mixed_sparse", "all_kinds_blocks"])
 def test_cg_solve(gpu_lib, oracle, name):
     coeffs, g, o, rng = make_pair(gpu_lib, oracle, name, seed=2)

@@ -47,6 +47,8 @@ CASES = {
     "c1_psdp": True,
     "c2_psdp": False,
     "c5_psdp": False,
+    # phase A stops with suspected infeasibility; the self-dual embedding ends it
+    "hsd_dense_n10_m15": True,
 }
 
 

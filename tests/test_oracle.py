@@ -75,6 +75,33 @@ def test_schur_closed_form(oracle, name):
     assert abs(t - np.trace(W)) <= 1e-12 * abs(np.trace(W))
 
 
+@pytest.mark.parametrize("name", list(CASES))
+def test_hsd_terms_closed_form(oracle, name):
+    """The self-dual embedding's terms of a HOMOGENEOUS build (B = W C W):
+    <A_i, B>, <C, W>, <C, B> and Rd tr(B), and the same M as an INFEASIBLE build."""
+    coeffs, d, rng, y, eye = make(oracle, name, 5)
+    d.update(1.0, -1.0, y, eye, 0)
+    assert d.factorize(0) == 1
+    Rd = -0.37
+    a0, b0, t0 = d.schur(0, Rd)
+    M0 = d.get_schur()
+    a, b, t = d.schur(2, Rd)
+    assert rel(d.get_schur(), M0) == 0.0 and rel(a, a0) == 0.0 and t == t0
+    aCS, cs, csc, csrd = d.hsd(Rd)
+    W = np.linalg.inv(dt.dense_buffer(coeffs, 1.0, -1.0, y, eye))
+    Cm = coeffs[0].dense()
+    B = W @ Cm @ W
+    A = [c.dense() for c in coeffs[1:]]
+    assert rel(aCS, [np.sum(Ai * B) for Ai in A]) < 1e-10
+    scale = max(1.0, np.abs(Cm).max() * np.abs(W).sum())
+    assert abs(cs - np.sum(Cm * W)) <= 1e-11 * scale
+    assert abs(csc - np.sum(Cm * B)) <= 1e-11 * max(1.0, abs(np.sum(Cm * B)))
+    assert abs(csrd - Rd * np.trace(B)) <= 1e-11 * max(1.0, abs(Rd * np.trace(B)))
+    if name == "all_kinds":
+        # a one-sided product (W C in place of W C W) is far outside the tolerance
+        assert rel(aCS, [np.sum(Ai * (W @ Cm)) for Ai in A]) > 1e-3
+
+
 def test_schur_catches_transposed_sandwich(oracle):
     """A plausible slip (W A_j instead of W A_j W) changes M far beyond the tolerance."""
     coeffs, d, rng, y, eye = make(oracle, "all_kinds", 1)
@@ -231,7 +258,7 @@ GOLDEN_CASES = ["theta_n50", "maxcut_n60", "dense_n20_m30", "dense_n40_m60", "c1
                 "dense_n120_m240", "rankone_n60_m500",
                 pytest.param("theta_n50_psdp", marks=pytest.mark.xfail(
                     reason="Lanczos warm-start rounding amplified past the reference's spread", strict=False)),
-                "maxcut_n60_psdp", "c1_psdp"]
+                "maxcut_n60_psdp", "c1_psdp", "hsd_dense_n10_m15"]
 
 
 @pytest.mark.parametrize("name", GOLDEN_CASES)
