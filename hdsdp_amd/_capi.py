@@ -58,6 +58,7 @@ _API = {
     "hdsdp_dev_ratio_test": (C.c_int, [P_VOID, C.c_int, P_DBL]),
     "hdsdp_dev_schur": (C.c_int, [P_VOID, C.c_int, C.c_double, P_DBL, P_DBL, P_DBL]),
     "hdsdp_dev_hsd": (C.c_int, [P_VOID, C.c_double, P_DBL, P_DBL, P_DBL, P_DBL]),
+    "hdsdp_dev_kkt_uses_ldl": (C.c_int, [P_VOID]),
     "hdsdp_dev_kkt_params": (C.c_int, [P_VOID, C.c_double, C.c_double, C.c_int, C.c_int]),
     "hdsdp_dev_kkt_add_diag": (C.c_int, [P_VOID, P_DBL]),
     "hdsdp_dev_kkt_regularize": (C.c_int, [P_VOID, C.c_double]),

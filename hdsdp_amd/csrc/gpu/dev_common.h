@@ -77,6 +77,14 @@ bool potrf(hipStream_t s, int n, T *A, int lda, T *work, size_t workLen, int *in
 template <typename T>
 void trtri(hipStream_t s, int n, const T *L, int ldl, T *Linv, int ldi, T *work);
 
+// In-place LU with partial pivoting, P A = L U (L unit lower, U upper, both in A),
+// blocked by 64 columns.  ipiv (device, m ints): row interchanged with row c at step c,
+// 0-based; info (device int, cleared by the caller): 0, or the first zero-pivot column + 1.
+void lu_factor(hipStream_t s, int m, double *A, int lda, int *ipiv, int *info);
+// x = A^-1 b from lu_factor's factors and perm, the composed interchanges (P b = b[perm]).
+// x and b are distinct m-vectors.
+void lu_solve(hipStream_t s, int m, const double *A, int lda, const int *perm, const double *b, double *x);
+
 // y[j] = alpha * sum_{i in rows(j)} A[i + j*lda] x[i] + beta * y[j] for j < ncols,
 // rows(j) = [0, m) (tri 0), [j, m) (tri 1, lower), [0, j] (tri 2, upper).
 template <typename T>

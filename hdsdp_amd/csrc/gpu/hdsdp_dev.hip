@@ -103,6 +103,12 @@ struct hdsdp_dev {
     bool wFromDual;     // W holds S^-1 of the last Schur build (not the primal method's X)
     double mixedFloor;  // mixed: refinement against the exact operator stops at this fraction of ||b||
     double *cb, *cxa;   // mixed refinement: right-hand side and accumulated solution
+    // the indefinite solver the Schur system switches to for good once its Cholesky
+    // factorisation or the CG it preconditions fails: LU of M with partial pivoting
+    bool useLU;
+    double *LUm;
+    int *luPiv, *luPerm, *luInfo;
+    std::vector<int> hpiv;
     int precKind;
     int useJacobi, paramUseJacobi, maxIter, nRestart;
     double absTol, relTol;
@@ -1492,23 +1498,85 @@ hdsdp_retcode hdsdp_dev_kkt_regularize(hdsdp_dev *d, double reg) {
     return check_launch();
 }
 
+// LU factorisation of the FP64 M into LUm (HFpLinsysSwitchToIndefinite,
+// linalg/hdsdp_linsolver.c:1236-1266: from the first call on, the system is solved
+// directly).  The reference factors M by LDL^T with Bunch-Kaufman pivoting (dsytrf); LU
+// with partial pivoting is as stable on a symmetric indefinite matrix, runs its trailing
+// updates on the FP64 GEMM, and solves the same system.
+hdsdp_retcode lu_factor(hdsdp_dev *d) {
+    const int m = d->m;
+    if (!d->LUm) {
+        DCALL(dalloc(d, &d->LUm, (size_t) m * m));
+        DCALL(dalloc(d, &d->luPiv, (size_t) m));
+        DCALL(dalloc(d, &d->luPerm, (size_t) m));
+        DCALL(dalloc(d, &d->luInfo, 1));
+        d->hpiv.assign((size_t) m + 1, 0);
+    }
+    d->useLU = true;
+    if (d->mGramPending) DCALL(rebuild_gram_fp64(d));
+    hdk::mirror_lower<double>(d->st, m, d->M, m);
+    HIP_OK(hipMemcpyAsync(d->LUm, d->M, sizeof(double) * (size_t) m * m, hipMemcpyDeviceToDevice, d->st));
+    HIP_OK(hipMemsetAsync(d->luInfo, 0, sizeof(int), d->st));
+    hdk::lu_factor(d->st, m, d->LUm, m, d->luPiv, d->luInfo);
+    DCALL(check_launch());
+    d->stats[4] += 1;
+    HIP_OK(hipMemcpyAsync(d->hpiv.data(), d->luPiv, sizeof(int) * m, hipMemcpyDeviceToHost, d->st));
+    HIP_OK(hipMemcpyAsync(d->hpiv.data() + m, d->luInfo, sizeof(int), hipMemcpyDeviceToHost, d->st));
+    DCALL(sync(d));
+    if (d->hpiv[m] != 0) return HDSDP_RETCODE_FAILED;
+    // compose the interchanges: (P b)[i] = b[perm[i]]
+    std::vector<int> perm(m);
+    for (int i = 0; i < m; ++i) perm[i] = i;
+    for (int c = 0; c < m; ++c) {
+        const int p = d->hpiv[c];
+        if (p < c || p >= m) return HDSDP_RETCODE_FAILED;
+        std::swap(perm[c], perm[p]);
+    }
+    HIP_OK(hipMemcpyAsync(d->luPerm, perm.data(), sizeof(int) * m, hipMemcpyHostToDevice, d->st));
+    return sync(d);
+}
+
+// HFpLinsysNumeric (linalg/hdsdp_linsolver.c:1367-1381): a Cholesky factorisation that
+// fails switches the system to the direct indefinite solver
 hdsdp_retcode hdsdp_dev_kkt_factorize(hdsdp_dev *d) {
+    if (d->useLU) return lu_factor(d);
     hdk::mirror_lower<double>(d->st, d->m, d->M, d->m);
     hdsdp_retcode rc = build_precond(d);
     if (rc != HDSDP_RETCODE_OK && d->mGramPending) {
         DCALL(rebuild_gram_fp64(d));
         rc = build_precond(d);
     }
+    if (rc == HDSDP_RETCODE_FAILED && !d->useJacobi) rc = lu_factor(d);
     return rc;
 }
 
+int hdsdp_dev_kkt_uses_ldl(hdsdp_dev *d) {
+    return d->useLU ? 1 : 0;
+}
+
+// HFpLinsysSolve (linalg/hdsdp_linsolver.c:1410-1425): a CG solve that fails switches the
+// system to the direct indefinite solver, which then solves this right-hand side
 hdsdp_retcode hdsdp_dev_kkt_solve(hdsdp_dev *d, const double *rhs, double *sol) {
     DCALL(upload(d, d->crhs, rhs, d->m, d->hvecB));
     prof_begin(d, HDSDP_PROF_CG);
-    int rc = cg_solve(d);
-    if (rc < 0 && d->mGramPending) {
-        DCALL(rebuild_gram_fp64(d));
-        if (build_precond(d) == HDSDP_RETCODE_OK) rc = cg_solve(d);
+    int rc = 0;
+    if (d->useLU) {
+        hdk::lu_solve(d->st, d->m, d->LUm, d->m, d->luPerm, d->crhs, d->cx);
+        DCALL(check_launch());
+    } else {
+        rc = cg_solve(d);
+        if (rc < 0 && d->mGramPending) {
+            DCALL(rebuild_gram_fp64(d));
+            if (build_precond(d) == HDSDP_RETCODE_OK) rc = cg_solve(d);
+        }
+        if (rc < 0) {
+            DCALL(upload(d, d->crhs, rhs, d->m, d->hvecB));
+            if (lu_factor(d) == HDSDP_RETCODE_OK) {
+                hdk::lu_solve(d->st, d->m, d->LUm, d->m, d->luPerm, d->crhs, d->cx);
+                DCALL(check_launch());
+                rc = 0;
+            }
+        }
     }
     DCALL(prof_end(d, HDSDP_PROF_CG));
     if (rc < 0) return HDSDP_RETCODE_FAILED;

@@ -69,12 +69,20 @@ hdsdp_retcode kkt_build_bound( hdsdp *s, int typeKKT ) {
     return hdsdp_dev_kkt_add_diag(s->sdp->dev, s->kkt->dBoundDiag);
 }
 
+/* HFpLinsysNumeric / HFpLinsysSolve (linalg/hdsdp_linsolver.c:1367-1425) log the
+ * switch of the Schur system to the indefinite solver */
 hdsdp_retcode kkt_factorize( hdsdp *s ) {
-    return hdsdp_dev_kkt_factorize(s->sdp->dev);
+    int before = hdsdp_dev_kkt_uses_ldl(s->sdp->dev);
+    hdsdp_retcode rc = hdsdp_dev_kkt_factorize(s->sdp->dev);
+    if ( !before && hdsdp_dev_kkt_uses_ldl(s->sdp->dev) ) hlog(s, "KKT system is almost indefinite. Switch to LDL. \n");
+    return rc;
 }
 
 hdsdp_retcode kkt_solve( hdsdp *s, const double *rhs, double *sol ) {
-    return hdsdp_dev_kkt_solve(s->sdp->dev, rhs, sol);
+    int before = hdsdp_dev_kkt_uses_ldl(s->sdp->dev);
+    hdsdp_retcode rc = hdsdp_dev_kkt_solve(s->sdp->dev, rhs, sol);
+    if ( !before && hdsdp_dev_kkt_uses_ldl(s->sdp->dev) ) hlog(s, "KKT system is unstable. Switch to LDL. \n");
+    return rc;
 }
 
 hdsdp_retcode kkt_regularize( hdsdp *s, double reg ) {

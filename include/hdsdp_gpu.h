@@ -127,6 +127,12 @@ hdsdp_retcode hdsdp_dev_kkt_params( hdsdp_dev *dev, double relTol, double absTol
                                     int maxIter, int nRestartFreq );
 /* M_ii += diag[i] (bound cone: interface/hdsdp_conic_bound.c:224-229). */
 hdsdp_retcode hdsdp_dev_kkt_add_diag( hdsdp_dev *dev, const double *diag );
+/* 1 once the Schur system has switched to the direct indefinite solver (a Cholesky
+ * factorisation of M, or the CG preconditioned by it, failed; the switch is for good, as
+ * HFpLinsysSwitchToIndefinite's, linalg/hdsdp_linsolver.c:1236-1266, 1367-1425), else 0.
+ * hdsdp_dev_kkt_factorize and hdsdp_dev_kkt_solve make the switch themselves; the
+ * host reads it to log it as the reference does. */
+int hdsdp_dev_kkt_uses_ldl( hdsdp_dev *dev );
 /* HKKTRegularize (interface/hdsdp_schur.c:348-373). */
 hdsdp_retcode hdsdp_dev_kkt_regularize( hdsdp_dev *dev, double dKKTReg );
 /* Preconditioner build: Jacobi diagonal or Cholesky of M once switched

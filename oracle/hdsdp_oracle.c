@@ -14,6 +14,7 @@
  *   Lanczos step length  linalg/hdsdp_lanczos.c:33-292, interface/hdsdp_conic_sdp.c:462-505, 1638-1682
  *   primal recovery      interface/hdsdp_conic_sdp.c:2328-2379
  *   self-dual terms      interface/hdsdp_conic_sdp.c:737-760, 983-1029
+ *   LDL^T fallback       linalg/hdsdp_linsolver.c:1115-1189, 1236-1266, 1367-1425
  *
  * Pinning: see tests/test_oracle.py (closed forms, brute-force traces on tiny
  * inputs, numpy/scipy factorizations) and the reference golden traces.
@@ -55,6 +56,11 @@ struct hdsdp_dev {
     double *r, *rnew, *p, *z, *Mp, *x, *rhsbuf;
     int useJacobi, paramUseJacobi, maxIter, nRestartFreq;
     double absTol, relTol;
+    /* the indefinite solver the Schur system switches to for good once its Cholesky
+     * factorisation or its Cholesky-preconditioned CG fails */
+    int useLDL;
+    double *Mldl;
+    int *ldlPiv;
     olanczos lz;
     /* primal method (interface/hdsdp_psdp.c): the iterate is X; Xs its starting value,
      * dXp the direction, XSX the buffer X S X (+ Xs dS Xs), Sold the dual matrix of the
@@ -378,6 +384,8 @@ void hdsdp_dev_destroy( hdsdp_dev **pdev ) {
         for ( int k = 0; k <= d->m; ++k ) { free(d->A[k].val); free(d->A[k].row); free(d->A[k].col); }
         free(d->A);
     }
+    free(d->Mldl);
+    free(d->ldlPiv);
     for ( int k = 0; k < 3; ++k ) free(d->S[k]);
     for ( int k = 0; k < 2; ++k ) free(d->L[k]);
     free(d->W); free(d->W2); free(d->X); free(d->tmp); free(d->tmp2); free(d->vecn);
@@ -720,8 +728,147 @@ static hdsdp_retcode build_precond( hdsdp_dev *d ) {
     return ( ochol(m, d->Mchol) == 0 ) ? HDSDP_RETCODE_OK : HDSDP_RETCODE_FAILED;
 }
 
+/* Symmetric indefinite LDL^T with Bunch-Kaufman diagonal pivoting on the lower triangle
+ * (Bunch & Kaufman 1977, as LAPACK's dsytf2 'L' states it), the factorisation of the
+ * reference's indefinite Schur solver (lapackIndefiniteLinSolverNumeric,
+ * linalg/hdsdp_linsolver.c:1115-1136, dsytrf).  ipiv as LAPACK's, 1-based: k + 1 for a 1 x 1
+ * pivot with row k interchanged, -(k + 1) on both rows of a 2 x 2 pivot.  Returns 0, or
+ * the column + 1 of a zero pivot. */
+static int oldl_factor( int n, double *A, int *ipiv ) {
+    const double alpha = (1.0 + sqrt(17.0)) / 8.0;
+    int info = 0, k = 0;
+    while ( k < n ) {
+        int kstep = 1, kp = k;
+        double absakk = fabs(FE(A, n, k, k)), colmax = 0.0;
+        int imax = k;
+        for ( int i = k + 1; i < n; ++i ) {
+            if ( fabs(FE(A, n, i, k)) > colmax ) { colmax = fabs(FE(A, n, i, k)); imax = i; }
+        }
+        if ( OMAX(absakk, colmax) == 0.0 ) {
+            if ( !info ) info = k + 1;
+        } else {
+            if ( absakk < alpha * colmax ) {
+                double rowmax = 0.0;
+                for ( int j = k; j < imax; ++j ) rowmax = OMAX(rowmax, fabs(FE(A, n, imax, j)));
+                for ( int i = imax + 1; i < n; ++i ) rowmax = OMAX(rowmax, fabs(FE(A, n, i, imax)));
+                if ( absakk >= alpha * colmax * (colmax / rowmax) ) {
+                    kp = k;
+                } else if ( fabs(FE(A, n, imax, imax)) >= alpha * rowmax ) {
+                    kp = imax;
+                } else {
+                    kp = imax;
+                    kstep = 2;
+                }
+            }
+            int kk = k + kstep - 1;
+            if ( kp != kk ) {
+                double t;
+                for ( int i = kp + 1; i < n; ++i ) { t = FE(A, n, i, kk); FE(A, n, i, kk) = FE(A, n, i, kp); FE(A, n, i, kp) = t; }
+                for ( int j = kk + 1; j < kp; ++j ) { t = FE(A, n, j, kk); FE(A, n, j, kk) = FE(A, n, kp, j); FE(A, n, kp, j) = t; }
+                t = FE(A, n, kk, kk); FE(A, n, kk, kk) = FE(A, n, kp, kp); FE(A, n, kp, kp) = t;
+                if ( kstep == 2 ) { t = FE(A, n, k + 1, k); FE(A, n, k + 1, k) = FE(A, n, kp, k); FE(A, n, kp, k) = t; }
+            }
+            if ( kstep == 1 ) {
+                double r1 = 1.0 / FE(A, n, k, k);
+                for ( int j = k + 1; j < n; ++j ) {
+                    double ajk = FE(A, n, j, k);
+                    for ( int i = j; i < n; ++i ) FE(A, n, i, j) -= r1 * FE(A, n, i, k) * ajk;
+                }
+                for ( int i = k + 1; i < n; ++i ) FE(A, n, i, k) *= r1;
+            } else if ( k < n - 2 ) {
+                double d21 = FE(A, n, k + 1, k);
+                double d11 = FE(A, n, k + 1, k + 1) / d21, d22 = FE(A, n, k, k) / d21;
+                double t = 1.0 / (d11 * d22 - 1.0);
+                d21 = t / d21;
+                for ( int j = k + 2; j < n; ++j ) {
+                    double wk = d21 * (d11 * FE(A, n, j, k) - FE(A, n, j, k + 1));
+                    double wkp1 = d21 * (d22 * FE(A, n, j, k + 1) - FE(A, n, j, k));
+                    for ( int i = j; i < n; ++i ) FE(A, n, i, j) -= FE(A, n, i, k) * wk + FE(A, n, i, k + 1) * wkp1;
+                    FE(A, n, j, k) = wk;
+                    FE(A, n, j, k + 1) = wkp1;
+                }
+            }
+        }
+        if ( kstep == 1 ) {
+            ipiv[k] = kp + 1;
+        } else {
+            ipiv[k] = ipiv[k + 1] = -(kp + 1);
+        }
+        k += kstep;
+    }
+    return info;
+}
+
+/* b <- A^-1 b from oldl_factor's L, D and ipiv (LAPACK's dsytrs 'L' steps:
+ * lapackIndefiniteLinSolverSolveN, linalg/hdsdp_linsolver.c:1170-1189) */
+static void oldl_solve( int n, const double *A, const int *ipiv, double *b ) {
+    double t;
+    int k = 0;
+    while ( k < n ) {
+        if ( ipiv[k] > 0 ) {
+            int kp = ipiv[k] - 1;
+            if ( kp != k ) { t = b[k]; b[k] = b[kp]; b[kp] = t; }
+            for ( int i = k + 1; i < n; ++i ) b[i] -= FE(A, n, i, k) * b[k];
+            b[k] /= FE(A, n, k, k);
+            k += 1;
+        } else {
+            int kp = -ipiv[k] - 1;
+            if ( kp != k + 1 ) { t = b[k + 1]; b[k + 1] = b[kp]; b[kp] = t; }
+            for ( int i = k + 2; i < n; ++i ) b[i] -= FE(A, n, i, k) * b[k] + FE(A, n, i, k + 1) * b[k + 1];
+            double akm1k = FE(A, n, k + 1, k);
+            double akm1 = FE(A, n, k, k) / akm1k, ak = FE(A, n, k + 1, k + 1) / akm1k;
+            double denom = akm1 * ak - 1.0;
+            double bkm1 = b[k] / akm1k, bk = b[k + 1] / akm1k;
+            b[k] = (ak * bkm1 - bk) / denom;
+            b[k + 1] = (akm1 * bk - bkm1) / denom;
+            k += 2;
+        }
+    }
+    k = n - 1;
+    while ( k >= 0 ) {
+        if ( ipiv[k] > 0 ) {
+            for ( int i = k + 1; i < n; ++i ) b[k] -= FE(A, n, i, k) * b[i];
+            int kp = ipiv[k] - 1;
+            if ( kp != k ) { t = b[k]; b[k] = b[kp]; b[kp] = t; }
+            k -= 1;
+        } else {
+            for ( int i = k + 1; i < n; ++i ) {
+                b[k] -= FE(A, n, i, k) * b[i];
+                b[k - 1] -= FE(A, n, i, k - 1) * b[i];
+            }
+            int kp = -ipiv[k] - 1;
+            if ( kp != k ) { t = b[k]; b[k] = b[kp]; b[kp] = t; }
+            k -= 2;
+        }
+    }
+}
+
+/* HFpLinsysSwitchToIndefinite (linalg/hdsdp_linsolver.c:1236-1266): from now on the
+ * Schur system is factored by LDL^T and solved directly */
+static hdsdp_retcode ldl_factor( hdsdp_dev *d ) {
+    int m = d->m;
+    if ( !d->Mldl ) {
+        d->Mldl = (double *) malloc(sizeof(double) * (size_t) m * m);
+        d->ldlPiv = (int *) malloc(sizeof(int) * (size_t) m);
+        if ( !d->Mldl || !d->ldlPiv ) return HDSDP_RETCODE_MEMORY;
+    }
+    d->useLDL = 1;
+    memcpy(d->Mldl, d->M, sizeof(double) * (size_t) m * m);
+    d->stats[4] += 1;
+    return ( oldl_factor(m, d->Mldl, d->ldlPiv) == 0 ) ? HDSDP_RETCODE_OK : HDSDP_RETCODE_FAILED;
+}
+
+/* HFpLinsysNumeric (linalg/hdsdp_linsolver.c:1367-1381): a Cholesky factorisation that
+ * fails switches the system to the indefinite solver */
 hdsdp_retcode hdsdp_dev_kkt_factorize( hdsdp_dev *d ) {
-    return build_precond(d);
+    if ( d->useLDL ) return ldl_factor(d);
+    hdsdp_retcode rc = build_precond(d);
+    if ( rc == HDSDP_RETCODE_FAILED && !d->useJacobi ) rc = ldl_factor(d);
+    return rc;
+}
+
+int hdsdp_dev_kkt_uses_ldl( hdsdp_dev *d ) {
+    return d->useLDL;
 }
 
 static void apply_precond( hdsdp_dev *d, double *v ) {
@@ -803,9 +950,20 @@ static hdsdp_retcode cg_solve( hdsdp_dev *d, const double *rhs, double *sol ) {
     return HDSDP_RETCODE_OK;
 }
 
+/* HFpLinsysSolve (linalg/hdsdp_linsolver.c:1410-1425): a CG solve that fails switches
+ * the system to the indefinite solver, which then solves this right-hand side */
 hdsdp_retcode hdsdp_dev_kkt_solve( hdsdp_dev *d, const double *rhs, double *sol ) {
-    memcpy(d->rhsbuf, rhs, sizeof(double) * d->m);
-    return cg_solve(d, d->rhsbuf, sol);
+    int m = d->m;
+    memcpy(d->rhsbuf, rhs, sizeof(double) * m);
+    if ( !d->useLDL ) {
+        hdsdp_retcode rc = cg_solve(d, d->rhsbuf, sol);
+        if ( rc != HDSDP_RETCODE_FAILED ) return rc;
+        memcpy(d->rhsbuf, rhs, sizeof(double) * m);
+        if ( ldl_factor(d) != HDSDP_RETCODE_OK ) return HDSDP_RETCODE_FAILED;
+    }
+    oldl_solve(m, d->Mldl, d->ldlPiv, d->rhsbuf);
+    memcpy(sol, d->rhsbuf, sizeof(double) * m);
+    return HDSDP_RETCODE_OK;
 }
 
 /* hdsdp_dev_kkt_solve_noalloc on host memory: the Jacobi-preconditioned CG of

@@ -225,6 +225,9 @@ class Dev:
         self._ok(rc, "kkt_solve")
         return sol
 
+    def uses_ldl(self):
+        return self.lib.hdsdp_dev_kkt_uses_ldl(self.h)
+
     def primal(self, mu, y, dy):
         y = np.ascontiguousarray(y, dtype=np.float64)
         dy = np.ascontiguousarray(dy, dtype=np.float64)
@@ -381,3 +384,43 @@ def adjoint_fd_check(lib, inst, n_dirs, seed=0, h=1e-6, relopt=1e-8, noise=False
     if noise:
         return r["errs"], r["mu"], r["dLdb"], r["dLdC"], r["y"], r["fd_noise"]
     return r["errs"], r["mu"], r["dLdb"], r["dLdC"], r["y"]
+
+
+def indefinite_fallback_run(d, coeffs, y, eye, rng, mode):
+    """Drive one device's Schur system into the indefinite solver and back out of it:
+    M is made indefinite through the diagonal (mode "shift": M - s I with s between two
+    middle eigenvalues; mode "zero_diag": the diagonal removed, which leaves Bunch-Kaufman
+    only 2 x 2 pivots at first), a Jacobi CG capped at three steps fails, its Cholesky
+    switch fails, and the system switches to the direct solver for good.  Returns the
+    solutions of three right-hand sides (the last after a second factorisation with a
+    changed diagonal), the matrices they solve, and the switch flag before and after."""
+    m = len(coeffs) - 1
+    d.update(1.0, -1.0, y, eye, 0)
+    assert d.factorize(0) == 1
+    d.schur(0, 0.0)
+    M = d.get_schur()
+    M = np.tril(M) + np.tril(M, -1).T
+    if mode == "shift":
+        lam = np.linalg.eigvalsh(M)
+        k = m // 2
+        delta = -0.5 * (lam[k] + lam[k + 1]) * np.ones(m)
+    else:
+        delta = -np.diag(M).copy()
+    d.add_diag(delta)
+    A1 = M + np.diag(delta)
+    d.kkt_params(1e-14, 1e-14, 3)
+    flag0 = d.uses_ldl()
+    assert d.kkt_factorize() == 0
+    sols, mats = [], []
+    for k in range(2):
+        rhs = rng.standard_normal(m)
+        sols.append((d.kkt_solve(rhs), rhs))
+        mats.append(A1)
+    delta2 = rng.uniform(-0.01, 0.01, m) * np.abs(np.diag(A1)).max()
+    d.add_diag(delta2)
+    A2 = A1 + np.diag(delta2)
+    assert d.kkt_factorize() == 0
+    rhs = rng.standard_normal(m)
+    sols.append((d.kkt_solve(rhs), rhs))
+    mats.append(A2)
+    return sols, mats, flag0, d.uses_ldl()

@@ -109,6 +109,27 @@ def test_hsd_terms_match_oracle_and_closed_form(gpu_lib, oracle, name):
         assert abs(got - ref) <= 1e-10 * scale and abs(got - orc) <= 1e-11 * scale
 
 
+# (a zero constraint leaves M a zero row: "zero_diag" runs on the dense case only)
+@pytest.mark.parametrize("name,mode", [("dense", "shift"), ("dense", "zero_diag"), ("all_kinds_blocks", "shift")])
+def test_indefinite_schur_switches_to_lu(gpu_lib, oracle, name, mode):
+    """The indefinite fallback: after the Cholesky switch fails on an indefinite M the
+    device solves by LU with partial pivoting, the oracle by Bunch-Kaufman LDL^T; both
+    match numpy's solve of that M, and each other."""
+    coeffs, g, o, rng = make_pair(gpu_lib, oracle, name, seed=7)
+    y, eye = dt.interior_point(coeffs, rng)
+    res = []
+    for d in (g, o):
+        r = np.random.default_rng(11)
+        res.append(dt.indefinite_fallback_run(d, coeffs, y, eye, r, mode))
+    (gs, gm, gf0, gf1), (os_, om, of0, of1) = res
+    assert (gf0, gf1) == (0, 1) and (of0, of1) == (0, 1)
+    for (gx, rhs), (ox, _), A in zip(gs, os_, gm):
+        ref = np.linalg.solve(A, rhs)
+        bound = 1e-15 * np.linalg.cond(A) * 100
+        assert np.linalg.norm(gx - ref) / np.linalg.norm(ref) <= bound
+        assert np.linalg.norm(gx - ox) / np.linalg.norm(ox) <= 2 * bound
+
+
 @pytest.mark.parametrize("name", ["spr1_unit", "mixed_sparse", "all_kinds_blocks"])
 def test_cg_solve(gpu_lib, oracle, name):
     coeffs, g, o, rng = make_pair(gpu_lib, oracle, name, seed=2)

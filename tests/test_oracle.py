@@ -102,6 +102,21 @@ def test_hsd_terms_closed_form(oracle, name):
         assert rel(aCS, [np.sum(Ai * (W @ Cm)) for Ai in A]) > 1e-3
 
 
+# (a zero constraint leaves M a zero row: "zero_diag" runs on the dense case only)
+@pytest.mark.parametrize("name,mode", [("dense", "shift"), ("dense", "zero_diag"), ("all_kinds", "shift")])
+def test_indefinite_schur_switches_to_ldl(oracle, name, mode):
+    """The LDL^T fallback (Bunch-Kaufman, as dsytrf/dsytrs): after the Cholesky switch
+    fails on an indefinite M, each solve matches numpy's solve of that M."""
+    coeffs, d, rng, y, eye = make(oracle, name, 7)
+    sols, mats, flag0, flag1 = dt.indefinite_fallback_run(d, coeffs, y, eye, rng, mode)
+    assert flag0 == 0 and flag1 == 1
+    for (x, rhs), A in zip(sols, mats):
+        ref = np.linalg.solve(A, rhs)
+        err = np.linalg.norm(x - ref) / np.linalg.norm(ref)
+        assert err <= 1e-15 * np.linalg.cond(A) * 100, (err, np.linalg.cond(A))
+        assert np.linalg.norm(A @ x - rhs) <= 1e-12 * np.linalg.norm(A, 2) * np.linalg.norm(x) * 10
+
+
 def test_schur_catches_transposed_sandwich(oracle):
     """A plausible slip (W A_j instead of W A_j W) changes M far beyond the tolerance."""
     coeffs, d, rng, y, eye = make(oracle, "all_kinds", 1)
