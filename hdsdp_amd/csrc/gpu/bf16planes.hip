@@ -185,8 +185,14 @@ __global__ __launch_bounds__(PTHREADS) void gemm_bf16p_kernel(int M, int N, int 
     // for the linear workgroup index g), and XCD c is given a run of consecutive
     // (batch entry, tile) pairs in batch-major order: whole batch entries per XCD, which
     // keeps each entry's operands in one L2 and gives every XCD the same work.
+    // Within a batch entry the tiles then run longest first (their k ranges differ by up
+    // to 8x): tri 3 by the distance of the tile from the diagonal, j - i, descending;
+    // tri 4 (lower tiles) by column, descending.  The tiles of the last entries an XCD
+    // runs are then short ones, not the longest.
     int gx = blockIdx.x, gy = blockIdx.y, bz = blockIdx.z;
     const bool batchOrder = (lin & 4) != 0;
+    bool mapped = false;
+    int mx = 0, my = 0;
     if (batchOrder) {
         const long tiles = (long) gridDim.x * gridDim.y, total = tiles * gridDim.z;
         const long g = gx + (long) gridDim.x * (gy + (long) gridDim.y * bz);
@@ -196,10 +202,46 @@ __global__ __launch_bounds__(PTHREADS) void gemm_bf16p_kernel(int M, int N, int 
         gy = (int) ((t % tiles) / gridDim.x);
         gx = (int) (t % gridDim.x);
         lin &= 3;
+        int tt = (int) (t % tiles);
+        if (tri == 3 && lin == 0 && gridDim.x == gridDim.y) {
+            const int mt = gridDim.x;
+            for (int d = mt - 1; d >= 0 && !mapped; --d) {  // upper tiles, j - i = d
+                if (tt < mt - d) {
+                    mx = tt;
+                    my = tt + d;
+                    mapped = true;
+                } else {
+                    tt -= mt - d;
+                }
+            }
+            for (int d = 1; d < mt && !mapped; ++d) {  // lower tiles (return at once)
+                if (tt < mt - d) {
+                    mx = tt + d;
+                    my = tt;
+                    mapped = true;
+                } else {
+                    tt -= mt - d;
+                }
+            }
+        } else if (tri == 4 && lin == 1 && gridDim.y == 1) {
+            const int mt = (M + PT - 1) / PT;
+            for (int j = mt - 1; j >= 0 && !mapped; --j) {  // column j: rows j .. mt - 1
+                if (tt < mt - j) {
+                    mx = j + tt;
+                    my = j;
+                    mapped = true;
+                } else {
+                    tt -= mt - j;
+                }
+            }
+        }
     }
     int bx = gx, by = gy;
     if (!lin && !batchOrder && (tri == 2 || tri == 3)) bx = (int) ((gx + gy + bz) % gridDim.x);
-    if (lin == 2) {
+    if (mapped) {
+        bx = mx;
+        by = my;
+    } else if (lin == 2) {
         lower_tile_xcd(gx, gridDim.x, (M + PT - 1) / PT, bx, by);
     } else if (lin) {
         const int t = gx;
