@@ -1389,7 +1389,9 @@ hdsdp_retcode hdsdp_dev_schur(hdsdp_dev *d, int typeKKT, double Rd, double *asin
         prof_begin(d, HDSDP_PROF_PAIRS);
         hdk::schur_pairs(d->st, d->c, d->W, d->M, m);
         DCALL(prof_end(d, HDSDP_PROF_PAIRS));
-        const bool mixed = d->schurMixed != 0;
+        // once the system has switched to the direct LU solve, M is factored in FP64 as
+        // built: no split products to rebuild
+        const bool mixed = d->schurMixed != 0 && !d->useLU;
         d->mGramPending = mixed ? 1 : 0;
         d->mVer += 1;
         if (mixed) {

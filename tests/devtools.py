@@ -393,7 +393,8 @@ def indefinite_fallback_run(d, coeffs, y, eye, rng, mode):
     only 2 x 2 pivots at first), a Jacobi CG capped at three steps fails, its Cholesky
     switch fails, and the system switches to the direct solver for good.  Returns the
     solutions of three right-hand sides (the last after a second factorisation with a
-    changed diagonal), the matrices they solve, and the switch flag before and after."""
+    changed diagonal), the matrices they solve, the switch flag before and after, and the
+    Schur matrix as the device built it."""
     m = len(coeffs) - 1
     d.update(1.0, -1.0, y, eye, 0)
     assert d.factorize(0) == 1
@@ -423,4 +424,4 @@ def indefinite_fallback_run(d, coeffs, y, eye, rng, mode):
     rhs = rng.standard_normal(m)
     sols.append((d.kkt_solve(rhs), rhs))
     mats.append(A2)
-    return sols, mats, flag0, d.uses_ldl()
+    return sols, mats, flag0, d.uses_ldl(), M

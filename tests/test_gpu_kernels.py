@@ -121,13 +121,35 @@ def test_indefinite_schur_switches_to_lu(gpu_lib, oracle, name, mode):
     for d in (g, o):
         r = np.random.default_rng(11)
         res.append(dt.indefinite_fallback_run(d, coeffs, y, eye, r, mode))
-    (gs, gm, gf0, gf1), (os_, om, of0, of1) = res
+    (gs, gm, gf0, gf1, _), (os_, om, of0, of1, _) = res
     assert (gf0, gf1) == (0, 1) and (of0, of1) == (0, 1)
     for (gx, rhs), (ox, _), A in zip(gs, os_, gm):
         ref = np.linalg.solve(A, rhs)
         bound = 1e-15 * np.linalg.cond(A) * 100
         assert np.linalg.norm(gx - ref) / np.linalg.norm(ref) <= bound
         assert np.linalg.norm(gx - ox) / np.linalg.norm(ox) <= 2 * bound
+
+
+def test_indefinite_schur_switch_in_mixed_precision(gpu_lib):
+    """The same switch on a mixed-precision device (dense constraints, whose Schur
+    blocks come from bf16 split products): the LU solve runs on the FP64 M."""
+    n, m, kinds = CASES["dense"]
+    rng = np.random.default_rng(7)
+    coeffs = dt.rand_coeffs(rng, n, m, kinds)
+    d = dt.Dev(gpu_lib, coeffs, precision=1)
+    y, eye = dt.interior_point(coeffs, rng)
+    sols, mats, f0, f1, Mread = dt.indefinite_fallback_run(d, coeffs, y, eye, np.random.default_rng(11), "shift")
+    assert (f0, f1) == (0, 1)
+    d64 = dt.Dev(gpu_lib, coeffs, precision=0)
+    d64.update(1.0, -1.0, y, eye, 0)
+    assert d64.factorize(0) == 1
+    d64.schur(0, 0.0)
+    M64 = d64.get_schur()
+    M64 = np.tril(M64) + np.tril(M64, -1).T
+    for (x, rhs), A in zip(sols, mats):
+        A = A - Mread + M64  # the same diagonal changes on the FP64 Schur matrix
+        ref = np.linalg.solve(A, rhs)
+        assert np.linalg.norm(x - ref) / np.linalg.norm(ref) <= 1e-15 * np.linalg.cond(A) * 100
 
 
 @pytest.mark.parametrize("name", ["spr1_unit", "mixed_sparse", "all_kinds_blocks"])

@@ -108,7 +108,7 @@ def test_indefinite_schur_switches_to_ldl(oracle, name, mode):
     """The LDL^T fallback (Bunch-Kaufman, as dsytrf/dsytrs): after the Cholesky switch
     fails on an indefinite M, each solve matches numpy's solve of that M."""
     coeffs, d, rng, y, eye = make(oracle, name, 7)
-    sols, mats, flag0, flag1 = dt.indefinite_fallback_run(d, coeffs, y, eye, rng, mode)
+    sols, mats, flag0, flag1, _ = dt.indefinite_fallback_run(d, coeffs, y, eye, rng, mode)
     assert flag0 == 0 and flag1 == 1
     for (x, rhs), A in zip(sols, mats):
         ref = np.linalg.solve(A, rhs)
