@@ -954,11 +954,10 @@ static hdsdp_retcode phase_hsd( hdsdp *s, int dOnly ) {
         absOpt = 1e+20;
         relOpt = 1e+20;
     }
-    (void) relOpt;
     double feasTol = HMIN(absFeas, relFeas * (1.0 + objOneNorm));
     feasTol = feasTol * objScal / sqrt(nSumCones);
     absOpt = absOpt * 1e-04;
-    relOpt = absOpt * 1e-04;  /* (from the absolute tolerance, as the reference sets it) */
+    relOpt = absOpt * 1e-04;  /* from the absolute tolerance, as interface/hdsdp_algo.c:401 sets it */
     if ( s->HStatus == HDSDP_UNKNOWN ) set_start_hsd(s, dOnly);
     print_header(s, ALGO_DUAL_HSD);
 
