@@ -70,6 +70,8 @@ CASES = {
     # the self-dual embedding runs until tau vanishes (interface/hdsdp_algo.c:1859-1875)
     "hsd_dense_n10_m15": (lambda: I.with_data(I.dense_random(10, 15, 7), C=-np.eye(10) * 5.0), {"relopt": 1e-6},
                           set()),
+    "hsd_dense_n60_m120": (lambda: I.with_data(I.dense_random(60, 120, 35), C=-np.eye(60) * 5.0), {"relopt": 1e-6},
+                           set()),
     # the primal method on (it starts once the dual gap is below 10% and ends the solve)
     "theta_n50_psdp": (lambda: I.lovasz_theta(50, 0.3, 21), {"psdp": 1}, set()),
     "maxcut_n60_psdp": (lambda: I.maxcut(60, 0.2, 22, "unif"), {"psdp": 1}, {"dense_primary"}),

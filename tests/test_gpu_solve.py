@@ -49,6 +49,9 @@ CASES = {
     "c5_psdp": False,
     # phase A stops with suspected infeasibility; the self-dual embedding ends it
     "hsd_dense_n10_m15": True,
+    # the same at n = 60, m = 120 (54 iterations); as tau vanishes the reference's own
+    # runs spread by 2.5% in y, recorded in the fixture
+    "hsd_dense_n60_m120": True,
 }
 
 

@@ -273,7 +273,7 @@ GOLDEN_CASES = ["theta_n50", "maxcut_n60", "dense_n20_m30", "dense_n40_m60", "c1
                 "dense_n120_m240", "rankone_n60_m500",
                 pytest.param("theta_n50_psdp", marks=pytest.mark.xfail(
                     reason="Lanczos warm-start rounding amplified past the reference's spread", strict=False)),
-                "maxcut_n60_psdp", "c1_psdp", "hsd_dense_n10_m15"]
+                "maxcut_n60_psdp", "c1_psdp", "hsd_dense_n10_m15", "hsd_dense_n60_m120"]
 
 
 @pytest.mark.parametrize("name", GOLDEN_CASES)
