@@ -104,7 +104,7 @@ def check_shadow(out, r, fix):
 
 
 @pytest.mark.parametrize("name", ["c1_gap1e-6", "theta_n50", "dense_n40_m60", "rankone_n60_m500",
-                                  "hsd_dense_n10_m15"])
+                                  "hsd_dense_n10_m15", "hsd_dense_n60_m120"])
 def test_oracle_abi_at_reference_iterates(name):
     r, calls, out, fix = run_glue(GLUE_ORACLE, name, shadow=True)
     check_shadow(out, r, fix)
@@ -112,7 +112,7 @@ def test_oracle_abi_at_reference_iterates(name):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["c1_gap1e-6", "c2_gap1e-6", "dense_n120_m240", "rankone_n60_m500",
-                                  "hsd_dense_n10_m15"])
+                                  "hsd_dense_n10_m15", "hsd_dense_n60_m120"])
 def test_device_abi_at_reference_iterates(name):
     """Every device slot call of a reference run, against the reference's own answer."""
     r, calls, out, fix = run_glue(GLUE_GPU, name, shadow=True)
